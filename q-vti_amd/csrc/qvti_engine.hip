@@ -1,0 +1,1182 @@
+// qvti_engine.hip -- host side of the MI355X engine: context, FP64-built
+// tables, launch sequence, HIP-graph stepping and the C ABI of include/qvti.h.
+//
+// Reference being replaced: the per-shot driver cuda_forward_acoustic_3D
+// (cuviscoelas_3D.cu:1137-1652) with its allocation/free helpers
+// (cuviscoelas_3D.cu:1655-2148) and setup kernels (cuviscoelas_3D.cu:22-410).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qvti.h"
+#include "qvti_internal.h"
+#include "qvti_kernels.h"
+
+using namespace qvti;
+
+// ------------------------------------------------------------ errors -----
+static thread_local std::string g_err;
+
+int qvti_fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+extern "C" const char* qvti_last_error(void) { return g_err.c_str(); }
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return qvti_fail(e_ == hipErrorOutOfMemory ? QVTI_E_NOMEM : QVTI_E_DEVICE,     \
+                             "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                    \
+    } while (0)
+
+// ------------------------------------------------------------ plans ------
+// Radix order: 4s first, then 2, 3, 5 (same factorisation the oracle uses).
+static bool make_plan(int n, FftPlan* p)
+{
+    if (n < 2 || n > kMaxN) return false;
+    int m = n, np = 0;
+    while (m % 4 == 0) { p->radix[np++] = 4; m /= 4; }
+    while (m % 2 == 0) { p->radix[np++] = 2; m /= 2; }
+    while (m % 3 == 0) { p->radix[np++] = 3; m /= 3; }
+    while (m % 5 == 0) { p->radix[np++] = 5; m /= 5; }
+    if (m != 1 || np > kMaxPasses) return false;
+    p->n = n;
+    p->npass = np;
+    return true;
+}
+
+// The reference's truncated pi (cuviscoelas_3D.cu:15).
+static const double kRefPi = 3.1415926;
+
+// D+ / D- symbols along one axis, cuviscoelas_3D.cu:44-149, with the FP32
+// intermediates the reference keeps (dk, tmp, K = tmp*m).  The Nyquist symbol
+// keeps its real part only (the `.x` of the reference's C2C result sees the
+// Hermitian part), and 1/n is folded in (the reference's `/ntp`).
+static void axis_symbols(int n, float d, std::vector<float2>& dp, std::vector<float2>& dm, std::vector<float>& kf)
+{
+    dp.resize(n);
+    dm.resize(n);
+    kf.resize(n);
+    const float dk = (float)(1.0 / n / d);
+    const float tmp = (float)(2 * kRefPi * dk);
+    const double inv = 1.0 / n;
+    for (int i = 0; i < n; ++i) {
+        const int m = i <= n / 2 ? i : n - i;
+        kf[i] = (float)(2 * kRefPi / n / d * m);
+        const float K = tmp * (float)m;
+        const double s = sin(m * kRefPi / n), c = cos(m * kRefPi / n);
+        float pr = (float)(-(double)K * s), mr = (float)((double)K * s);
+        float pim = (float)((double)K * c), mim = (float)((double)K * c);
+        if (i > n / 2) { pim = -pim; mim = -mim; }
+        if (2 * i == n) { pim = 0.f; mim = 0.f; }
+        dp[i] = make_float2((float)(pr * inv), (float)(pim * inv));
+        dm[i] = make_float2((float)(mr * inv), (float)(mim * inv));
+    }
+}
+
+// C-PML a(x), b(x), cuviscoelas_3D.cu:213-309: R = 1e-2, n1=2, n2=1, n3=2,
+// vmax forced to 5000 (:235), kappa = 1; plus half-cell averages.
+static void pml_profile(int n, int pml, float d, float f0, std::vector<float>& a, std::vector<float>& b,
+                        std::vector<float>& ah, std::vector<float>& bh)
+{
+    a.resize(n);
+    b.resize(n);
+    ah.resize(n);
+    bh.resize(n);
+    const int n1 = 2, n2 = 1, n3 = 2;
+    const float R = 1e-2f, vmax = 5000.0f;
+    const float alpha_max = (float)(kRefPi * f0);
+    const float om_max = (float)((1 + n1 + n2) * vmax * log(1.0 / R) / ((n1 + n2 - 1) * pml * d));
+    for (int i = 0; i < n; ++i) {
+        const int j = i < pml ? i : (i > n - 1 - pml ? n - 1 - i : -1);
+        float alpha, om;
+        if (j >= 0) {
+            alpha = alpha_max * powf((float)(1.0 * j / (pml - 1)), (float)n3);
+            om = om_max * powf((float)(1.0 * (pml - 1 - j) / pml), (float)(n1 + n2));
+        } else {
+            alpha = alpha_max;
+            om = 0.f;
+        }
+        a[i] = alpha + om / 1.0f;
+        b[i] = om / 1.0f;
+    }
+    for (int i = 0; i < n; ++i) {
+        ah[i] = i + 1 < n ? 0.5f * (a[i] + a[i + 1]) : a[i];
+        bh[i] = i + 1 < n ? 0.5f * (b[i] + b[i + 1]) : b[i];
+    }
+}
+
+static bool supported_n(int n) { FftPlan p; return n >= 8 && n % 2 == 0 && make_plan(n, &p); }
+
+// ------------------------------------------------------------ context ----
+namespace {
+enum KClass {
+    KC_TICK = 0, KC_VEL_X, KC_VEL_Y, KC_VEL_Z, KC_Z_FWD, KC_Y_FWD, KC_XPASS, KC_Y_INV, KC_Z_INV,
+    KC_STR_X, KC_STR_Y, KC_STR_Z, KC_RECORD, KC_SNAP, KC_PROBE, KC_COUNT
+};
+const char* kClassNames[KC_COUNT] = {"tick", "vel_x", "vel_y", "vel_z", "z_fwd", "y_fwd", "xpass", "y_inv",
+                                     "z_inv", "str_x", "str_y", "str_z", "record", "snapshot", "probe"};
+
+struct AxisDev {
+    int n;
+    float d;
+    FftPlan plan;
+    float2 *tw = nullptr, *dp = nullptr, *dm = nullptr;
+    float *kf = nullptr, *a = nullptr, *b = nullptr, *ah = nullptr, *bh = nullptr;
+};
+}  // namespace
+
+struct qvti_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    qvti_grid g{};
+    long long N = 0, nrows = 0;
+    int S = 1;  // max shots
+    AxisDev ax[3];
+    // model
+    float* coef = nullptr;  // sets P, S, (H, D): [nsets*5][N]
+    float* sbar = nullptr;  // [5][N]
+    float* buoy = nullptr;  // [3][N]
+    bool has_model = false;
+    // state, field-major [field][shot][N]
+    float *v = nullptr, *sig = nullptr, *dv = nullptr, *H = nullptr, *bufA = nullptr, *bufB = nullptr;
+    float *phiv[3] = {nullptr, nullptr, nullptr}, *phis[3] = {nullptr, nullptr, nullptr};
+    long long slab[3] = {0, 0, 0};
+    int nphis[3] = {5, 5, 4};
+    // run
+    int nt = 0, nshots = 0, steps_done = 0, begun = 0;
+    float* d_rik = nullptr;
+    int* d_it = nullptr;
+    int* d_probe = nullptr;
+    int* d_src_kind = nullptr;
+    long long* d_src_ip = nullptr;
+    long long* d_rec_ip = nullptr;
+    int* d_rec_off = nullptr;
+    int total_rec = 0;
+    float* d_rec = nullptr;
+    long long rec_cap = 0;
+    int probe_every = 50;
+    float probe_limit = 1e30f;
+    // snapshot
+    int snap_field = 0, snap_every = 0;
+    float* snap_host = nullptr;
+    long long snap_cap = 0;
+    float* d_snap = nullptr;
+    long long d_snap_cap = 0;
+    // graphs
+    bool use_graphs = true;
+    hipGraphExec_t gexec = nullptr;
+    // timing
+    bool timing = false;
+    int ev_max = 0, ev_used = 0;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_class;
+    long long kc_launches[KC_COUNT] = {};
+    double kc_ms[KC_COUNT] = {};
+    // launch configuration
+    StridedArgs a_vx{}, a_vy{}, a_sx{}, a_sy{};
+    VelZArgs a_vz{};
+    ZRowsArgs a_zf{}, a_zi{};
+    CStridedArgs a_yf{}, a_yi{};
+    XPassArgs a_xp{};
+    StressZArgs a_sz{};
+    RecArgs a_rec{};
+    dim3 g_vx, g_vy, g_vz, g_zr, g_cy, g_xp, g_sz, g_rec;
+    long long bytes = 0;
+    std::vector<void*> allocs;
+};
+
+static int dalloc(qvti_ctx* c, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return qvti_fail(QVTI_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    c->allocs.push_back(*p);
+    c->bytes += (long long)bytes;
+    return QVTI_OK;
+}
+
+template <class T>
+static int upload(qvti_ctx* c, T** dst, const std::vector<T>& src)
+{
+    int rc = dalloc(c, (void**)dst, sizeof(T) * src.size());
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    return QVTI_OK;
+}
+
+static void release(qvti_ctx* c)
+{
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    for (auto e : c->ev) hipEventDestroy(e);
+    c->ev.clear();
+    for (void* p : c->allocs) hipFree(p);
+    c->allocs.clear();
+    if (c->stream) hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+}
+
+// ------------------------------------------------------- sizes / memory --
+static long long state_fields() { return 3 + 6 + 3 + 9 + 15 + 15; }
+
+extern "C" long long qvti_estimate_bytes(const qvti_grid* g, int max_shots)
+{
+    if (!g || max_shots < 1) return -1;
+    const long long N = (long long)g->nx * g->ny * g->nz;
+    const long long p2 = 2LL * g->pml;
+    const long long slabs = 8LL * (p2 * g->ny * g->nz + (long long)g->nx * p2 * g->nz) + 7LL * g->nx * g->ny * p2;
+    const long long model = (g->vti ? 20 : 10) + 5 + 3;
+    return 4LL * (N * (state_fields() * max_shots + model) + slabs * max_shots) + (64LL << 20);
+}
+
+extern "C" int qvti_device_count(int* n)
+{
+    if (!n) return qvti_fail(QVTI_E_ARG, "n is NULL");
+    int k = 0;
+    hipError_t e = hipGetDeviceCount(&k);
+    if (e != hipSuccess) {
+        *n = 0;
+        return qvti_fail(QVTI_E_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *n = k;
+    return QVTI_OK;
+}
+
+extern "C" long long qvti_device_bytes(qvti_ctx* c) { return c ? c->bytes : -1; }
+
+static int pick_pc(int cap, int n, int limit)
+{
+    int p = cap / n;
+    if (p > 64) p = 64;
+    if (p > limit) p = limit;
+    if (p < 1) p = 1;
+    return p;
+}
+
+static int build_launch_config(qvti_ctx* c);
+
+extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, int device)
+{
+    if (!out || !g) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    *out = nullptr;
+    if (max_shots < 1 || max_shots > 4096) return qvti_fail(QVTI_E_ARG, "max_shots %d out of range", max_shots);
+    const int dims[3] = {g->nx, g->ny, g->nz};
+    for (int a = 0; a < 3; ++a)
+        if (!supported_n(dims[a]))
+            return qvti_fail(QVTI_E_UNSUPPORTED,
+                             "axis %d length %d: the engine needs even 2/3/5-smooth lengths in [8, %d]", a, dims[a], kMaxN);
+    if (g->pml < 2 || 2 * g->pml > g->nx || 2 * g->pml > g->ny || 2 * g->pml > g->nz)
+        return qvti_fail(QVTI_E_ARG, "pml %d invalid for grid %dx%dx%d", g->pml, g->nx, g->ny, g->nz);
+    if (!(g->dx > 0 && g->dy > 0 && g->dz > 0 && g->dt > 0 && g->w0 > 0 && g->f0 > 0))
+        return qvti_fail(QVTI_E_ARG, "dx, dy, dz, dt, f0, w0 must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return qvti_fail(QVTI_E_DEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return qvti_fail(QVTI_E_ARG, "device %d of %d", device, ndev);
+    HIPCHK(hipSetDevice(device));
+
+    qvti_ctx* c = new qvti_ctx();
+    c->device = device;
+    c->g = *g;
+    c->S = max_shots;
+    c->N = (long long)g->nx * g->ny * g->nz;
+    c->nrows = (long long)g->nx * g->ny;
+    int rc = QVTI_OK;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return qvti_fail(QVTI_E_DEVICE, "hipStreamCreate failed");
+    }
+    const float ds[3] = {g->dx, g->dy, g->dz};
+    for (int a = 0; a < 3 && !rc; ++a) {
+        AxisDev& A = c->ax[a];
+        A.n = dims[a];
+        A.d = ds[a];
+        make_plan(A.n, &A.plan);
+        std::vector<float2> tw(A.n), dp, dm;
+        for (int t = 0; t < A.n; ++t) {
+            const double ang = -2.0 * M_PI * t / A.n;
+            tw[t] = make_float2((float)cos(ang), (float)sin(ang));
+        }
+        std::vector<float> kf, pa, pb, pah, pbh;
+        axis_symbols(A.n, A.d, dp, dm, kf);
+        pml_profile(A.n, g->pml, A.d, g->f0, pa, pb, pah, pbh);
+        rc = rc ? rc : upload(c, &A.tw, tw);
+        rc = rc ? rc : upload(c, &A.dp, dp);
+        rc = rc ? rc : upload(c, &A.dm, dm);
+        rc = rc ? rc : upload(c, &A.kf, kf);
+        rc = rc ? rc : upload(c, &A.a, pa);
+        rc = rc ? rc : upload(c, &A.b, pb);
+        rc = rc ? rc : upload(c, &A.ah, pah);
+        rc = rc ? rc : upload(c, &A.bh, pbh);
+    }
+    const long long N = c->N, S = c->S;
+    const int p2 = 2 * g->pml;
+    c->slab[0] = (long long)p2 * g->ny * g->nz;
+    c->slab[1] = (long long)g->nx * p2 * g->nz;
+    c->slab[2] = (long long)g->nx * g->ny * p2;
+    const int nsets = g->vti ? 4 : 2;
+    rc = rc ? rc : dalloc(c, (void**)&c->coef, sizeof(float) * nsets * 5 * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->sbar, sizeof(float) * 5 * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->buoy, sizeof(float) * 3 * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->v, sizeof(float) * 3 * S * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->sig, sizeof(float) * 6 * S * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->dv, sizeof(float) * 3 * S * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->H, sizeof(float) * 9 * S * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->bufA, sizeof(float) * 15 * S * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->bufB, sizeof(float) * 15 * S * N);
+    for (int a = 0; a < 3 && !rc; ++a) {
+        rc = dalloc(c, (void**)&c->phiv[a], sizeof(float) * 3 * S * c->slab[a]);
+        rc = rc ? rc : dalloc(c, (void**)&c->phis[a], sizeof(float) * c->nphis[a] * S * c->slab[a]);
+    }
+    rc = rc ? rc : dalloc(c, (void**)&c->d_it, sizeof(int));
+    rc = rc ? rc : dalloc(c, (void**)&c->d_probe, sizeof(int));
+    rc = rc ? rc : dalloc(c, (void**)&c->d_src_kind, sizeof(int) * S);
+    rc = rc ? rc : dalloc(c, (void**)&c->d_src_ip, sizeof(long long) * S);
+    rc = rc ? rc : dalloc(c, (void**)&c->d_rec_off, sizeof(int) * (S + 1));
+    if (!rc) rc = build_launch_config(c);
+    if (rc) {
+        std::string keep = g_err;
+        release(c);
+        delete c;
+        g_err = keep;
+        return rc;
+    }
+    *out = c;
+    return QVTI_OK;
+}
+
+extern "C" int qvti_destroy(qvti_ctx* c)
+{
+    if (!c) return QVTI_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    release(c);
+    delete c;
+    return QVTI_OK;
+}
+
+// Pointers into the field-major state for shot 0.
+static float* fld(float* base, int k, const qvti_ctx* c) { return base + (long long)k * c->S * c->N; }
+
+static int build_launch_config(qvti_ctx* c)
+{
+    const qvti_grid& g = c->g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long sx = (long long)ny * nz, sy = nz, N = c->N;
+    const AxisDev &X = c->ax[0], &Y = c->ax[1], &Z = c->ax[2];
+    const int nsets = g.vti ? 4 : 2;
+    (void)nsets;
+
+    float* acc[3] = {fld(c->bufA, 0, c), fld(c->bufA, 1, c), fld(c->bufA, 2, c)};
+    float* vv[3] = {fld(c->v, 0, c), fld(c->v, 1, c), fld(c->v, 2, c)};
+    float* ww[3] = {fld(c->dv, 0, c), fld(c->dv, 1, c), fld(c->dv, 2, c)};
+    // sig order: pxx pyy pzz pxy pxz pyz
+    float* sg[6];
+    for (int k = 0; k < 6; ++k) sg[k] = fld(c->sig, k, c);
+    float *pxx = sg[0], *pyy = sg[1], *pzz = sg[2], *pxy = sg[3], *pxz = sg[4], *pyz = sg[5];
+
+    // ---- velocity x: pxx (D-, half) -> acc_x, pxy (D+) -> acc_y, pxz (D+) -> acc_z
+    auto strided_common = [&](StridedArgs& a, int axis) {
+        const AxisDev& A = c->ax[axis];
+        a.nx = nx; a.ny = ny; a.nz = nz; a.pml = g.pml;
+        a.sx = sx; a.sy = sy;
+        a.axis = axis;
+        a.n = A.n;
+        a.Pc = pick_pc(kTileMax, A.n, nz / 2);
+        a.ztiles = (nz + 2 * a.Pc - 1) / (2 * a.Pc);
+        a.dt = g.dt;
+        a.shot_stride = N;
+        a.slab_stride = c->slab[axis];
+        a.plan = A.plan;
+        a.tw = A.tw;
+        a.sym[0] = A.dp;
+        a.sym[1] = A.dm;
+        a.pa = A.a; a.pb = A.b; a.pah = A.ah; a.pbh = A.bh;
+    };
+    auto job = [](const float* in, float* out, const float* accp, float* phi, int sym, int half) {
+        DJob j;
+        j.in = in; j.out = out; j.acc = accp; j.phi = phi; j.sym = sym; j.half = half;
+        return j;
+    };
+    StridedArgs& vx = c->a_vx;
+    strided_common(vx, 0);
+    vx.mode = MODE_VEL_FIRST;
+    vx.njobs = 3;
+    vx.job[0] = job(pxx, acc[0], nullptr, nullptr, 1, 1);
+    vx.job[1] = job(pxy, acc[1], nullptr, nullptr, 0, 0);
+    vx.job[2] = job(pxz, acc[2], nullptr, nullptr, 0, 0);
+    // slab arrays are [field][shot][slab]
+    for (int k = 0; k < 3; ++k) vx.job[k].phi = c->phiv[0] + (long long)k * c->S * c->slab[0];
+    c->g_vx = dim3(ny * vx.ztiles, 1, 1);
+
+    StridedArgs& vy = c->a_vy;
+    strided_common(vy, 1);
+    vy.mode = MODE_VEL_ACC;
+    vy.njobs = 3;
+    vy.job[0] = job(pxy, acc[0], acc[0], nullptr, 0, 0);
+    vy.job[1] = job(pyy, acc[1], acc[1], nullptr, 1, 1);
+    vy.job[2] = job(pyz, acc[2], acc[2], nullptr, 0, 0);
+    for (int k = 0; k < 3; ++k) vy.job[k].phi = c->phiv[1] + (long long)k * c->S * c->slab[1];
+    c->g_vy = dim3(nx * vy.ztiles, 1, 1);
+
+    // ---- velocity z + update: pxz (D+) -> vx, pyz (D+) -> vy, pzz (D-, half) -> vz
+    VelZArgs& vz = c->a_vz;
+    vz.nx = nx; vz.ny = ny; vz.nz = nz; vz.pml = g.pml;
+    vz.Pc = pick_pc(kTileMax, nz, (int)((c->nrows + 1) / 2));
+    vz.nrows = c->nrows;
+    vz.dt = g.dt;
+    vz.shot_stride = N;
+    vz.slab_stride = c->slab[2];
+    vz.plan = Z.plan; vz.tw = Z.tw; vz.sym[0] = Z.dp; vz.sym[1] = Z.dm;
+    vz.pa = Z.a; vz.pb = Z.b; vz.pah = Z.ah; vz.pbh = Z.bh;
+    vz.sig[0] = pxz; vz.sig[1] = pyz; vz.sig[2] = pzz;
+    vz.symsel[0] = 0; vz.symsel[1] = 0; vz.symsel[2] = 1;
+    vz.half[0] = 0; vz.half[1] = 0; vz.half[2] = 1;
+    for (int k = 0; k < 3; ++k) {
+        vz.acc[k] = acc[k];
+        vz.v[k] = vv[k];
+        vz.dv[k] = ww[k];
+        vz.buoy[k] = c->buoy + (long long)k * N;
+        vz.phi[k] = c->phiv[2] + (long long)k * c->S * c->slab[2];
+    }
+    vz.src_kind = c->d_src_kind;
+    vz.src_ip = c->d_src_ip;
+    vz.rik = nullptr;  // set at begin
+    vz.it = c->d_it;
+    c->g_vz = dim3((unsigned)((c->nrows + 2 * vz.Pc - 1) / (2 * vz.Pc)), 1, 1);
+
+    // ---- 3-D pipeline.  P_c = v_c + i dv_c (bufB as complex), Q1_c, Q3_c (bufA as complex)
+    float2* P[3];
+    float2* Q1[3];
+    float2* Q3[3];
+    for (int k = 0; k < 3; ++k) {
+        P[k] = reinterpret_cast<float2*>(fld(c->bufB, 2 * k, c));
+        Q1[k] = reinterpret_cast<float2*>(fld(c->bufA, 2 * k, c));
+        Q3[k] = reinterpret_cast<float2*>(fld(c->bufA, 6 + 2 * k, c));
+    }
+    // H: [A1 v x,y,z][A1 dv x,y,z][A3 v x,y,z]
+    float* H1v[3];
+    float* H1w[3];
+    float* H3v[3];
+    for (int k = 0; k < 3; ++k) {
+        H1v[k] = fld(c->H, k, c);
+        H1w[k] = fld(c->H, 3 + k, c);
+        H3v[k] = fld(c->H, 6 + k, c);
+    }
+    ZRowsArgs& zf = c->a_zf;
+    zf.nz = nz;
+    zf.Pc = pick_pc(kTileMax, nz, (int)c->nrows);
+    zf.sign = -1.f;
+    zf.nrows = c->nrows;
+    zf.shot_stride = N;
+    zf.plan = Z.plan; zf.tw = Z.tw;
+    zf.mode = 0;
+    zf.nfields = 3;
+    for (int k = 0; k < 3; ++k) { zf.re_in[k] = vv[k]; zf.im_in[k] = ww[k]; zf.cout[k] = P[k]; }
+    c->g_zr = dim3((unsigned)((c->nrows + zf.Pc - 1) / zf.Pc), 1, 1);
+
+    CStridedArgs& yf = c->a_yf;
+    yf.nz = nz; yf.sx = sx; yf.sy = sy;
+    yf.axis = 1; yf.n = ny;
+    yf.Pc = pick_pc(kTileMax, ny, nz);
+    yf.ztiles = (nz + yf.Pc - 1) / yf.Pc;
+    yf.sign = -1.f;
+    yf.shot_stride = N;
+    yf.plan = Y.plan; yf.tw = Y.tw;
+    yf.nfields = 3;
+    for (int k = 0; k < 3; ++k) yf.f[k] = P[k];
+    c->g_cy = dim3(nx * yf.ztiles, 1, 1);
+
+    XPassArgs& xp = c->a_xp;
+    xp.nx = nx; xp.ny = ny; xp.nz = nz; xp.sx = sx; xp.sy = sy;
+    xp.Pc = pick_pc(kTileMax / 2, nx, nz);
+    xp.ztiles = (nz + xp.Pc - 1) / xp.Pc;
+    xp.shot_stride = N;
+    xp.plan = X.plan; xp.tw = X.tw;
+    xp.kfx = X.kf; xp.kfy = Y.kf; xp.kfz = Z.kf;
+    xp.invN = 1.0 / (double)N;
+    xp.ncomp = 3;
+    for (int k = 0; k < 3; ++k) { xp.P[k] = P[k]; xp.Q1[k] = Q1[k]; xp.Q3[k] = Q3[k]; }
+    c->g_xp = dim3(ny * xp.ztiles, 1, 1);
+
+    CStridedArgs& yi = c->a_yi;
+    yi = yf;
+    yi.sign = 1.f;
+    yi.nfields = 6;
+    for (int k = 0; k < 3; ++k) { yi.f[k] = Q1[k]; yi.f[3 + k] = Q3[k]; }
+
+    ZRowsArgs& zi = c->a_zi;
+    zi = zf;
+    zi.mode = 1;
+    zi.sign = 1.f;
+    zi.nfields = 6;
+    for (int k = 0; k < 3; ++k) {
+        zi.cin[k] = Q1[k]; zi.re_out[k] = H1v[k]; zi.im_out[k] = H1w[k];
+        zi.cin[3 + k] = Q3[k]; zi.re_out[3 + k] = H3v[k]; zi.im_out[3 + k] = nullptr;
+    }
+
+    // ---- stress x / y derivatives of the 15 family fields
+    const float* fam[3][5];
+    for (int k = 0; k < 3; ++k) {
+        fam[k][0] = H1v[k]; fam[k][1] = vv[k]; fam[k][2] = H3v[k]; fam[k][3] = H1w[k]; fam[k][4] = ww[k];
+    }
+    float* DX[15];
+    float* DY[15];
+    for (int k = 0; k < 15; ++k) { DX[k] = fld(c->bufA, k, c); DY[k] = fld(c->bufB, k, c); }
+    StridedArgs& sxa = c->a_sx;
+    strided_common(sxa, 0);
+    sxa.mode = MODE_STORE;
+    sxa.njobs = 15;
+    for (int j = 0; j < 5; ++j) {
+        sxa.job[j] = job(fam[0][j], DX[j], nullptr, nullptr, 0, 0);        // ex: dx vx-family, D+
+        sxa.job[5 + j] = job(fam[2][j], DX[5 + j], nullptr, nullptr, 1, 0);   // gx: dx vz-family, D-
+        sxa.job[10 + j] = job(fam[1][j], DX[10 + j], nullptr, nullptr, 1, 0); // hx: dx vy-family, D-
+    }
+    StridedArgs& sya = c->a_sy;
+    strided_common(sya, 1);
+    sya.mode = MODE_STORE;
+    sya.njobs = 15;
+    for (int j = 0; j < 5; ++j) {
+        sya.job[j] = job(fam[1][j], DY[j], nullptr, nullptr, 0, 0);        // ey: dy vy-family, D+
+        sya.job[5 + j] = job(fam[2][j], DY[5 + j], nullptr, nullptr, 1, 0);   // gy: dy vz-family, D-
+        sya.job[10 + j] = job(fam[0][j], DY[10 + j], nullptr, nullptr, 1, 0); // hy: dy vx-family, D-
+    }
+
+    // ---- stress z + update
+    StressZArgs& sz = c->a_sz;
+    sz.nx = nx; sz.ny = ny; sz.nz = nz; sz.pml = g.pml;
+    sz.Pc = std::max(1, std::min(std::min(512 / nz, 32), (int)((c->nrows + 1) / 2)));
+    sz.nrows = c->nrows;
+    sz.dt = g.dt;
+    sz.shot_stride = N;
+    sz.slabx_stride = c->slab[0]; sz.slaby_stride = c->slab[1]; sz.slabz_stride = c->slab[2];
+    sz.plan = Z.plan; sz.tw = Z.tw; sz.symp = Z.dp; sz.symm = Z.dm;
+    sz.pax = X.a; sz.pbx = X.b; sz.paxh = X.ah; sz.pbxh = X.bh;
+    sz.pay = Y.a; sz.pby = Y.b; sz.payh = Y.ah; sz.pbyh = Y.bh;
+    sz.paz = Z.a; sz.pbz = Z.b; sz.pazh = Z.ah; sz.pbzh = Z.bh;
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 5; ++j) sz.fam[k][j] = fam[k][j];
+    for (int j = 0; j < 5; ++j) {
+        sz.ex[j] = DX[j]; sz.gx[j] = DX[5 + j]; sz.hx[j] = DX[10 + j];
+        sz.ey[j] = DY[j]; sz.gy[j] = DY[5 + j]; sz.hy[j] = DY[10 + j];
+        const float* Pset = c->coef + (long long)(0 * 5 + j) * N;
+        const float* Sset = c->coef + (long long)(1 * 5 + j) * N;
+        const float* Hset = g.vti ? c->coef + (long long)(2 * 5 + j) * N : Pset;
+        const float* Dset = g.vti ? c->coef + (long long)(3 * 5 + j) * N : Pset;
+        sz.c11[j] = Hset; sz.c33[j] = Pset; sz.c13[j] = Dset;
+        sz.cs[j] = Sset; sz.csb[j] = c->sbar + (long long)j * N;
+    }
+    for (int k = 0; k < 6; ++k) sz.sig[k] = sg[k];
+    for (int k = 0; k < 5; ++k) {
+        sz.phx[k] = c->phis[0] + (long long)k * c->S * c->slab[0];
+        sz.phy[k] = c->phis[1] + (long long)k * c->S * c->slab[1];
+    }
+    for (int k = 0; k < 4; ++k) sz.phz[k] = c->phis[2] + (long long)k * c->S * c->slab[2];
+    sz.src_kind = c->d_src_kind;
+    sz.src_ip = c->d_src_ip;
+    sz.rik = nullptr;
+    sz.it = c->d_it;
+    c->g_sz = dim3((unsigned)((c->nrows + 2 * sz.Pc - 1) / (2 * sz.Pc)), 1, 1);
+
+    RecArgs& rec = c->a_rec;
+    rec.vx = vv[0]; rec.vy = vv[1]; rec.vz = vv[2];
+    rec.shot_stride = N;
+    rec.rec_off = c->d_rec_off;
+    rec.it = c->d_it;
+    (void)pxx; (void)pyy;
+    return QVTI_OK;
+}
+
+// ------------------------------------------------------------ model ------
+extern "C" int qvti_set_model(qvti_ctx* c, const float* velp, const float* gama_p, const float* vels,
+                              const float* gama_s, const float* rho, const float* eps, const float* delta)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    if (!velp || !gama_p || !vels || !gama_s || !rho) return qvti_fail(QVTI_E_ARG, "model array is NULL");
+    if (c->g.vti && (!eps || !delta)) return qvti_fail(QVTI_E_ARG, "VTI grid needs eps and delta");
+    HIPCHK(hipSetDevice(c->device));
+    const long long N = c->N;
+    const size_t B = sizeof(float) * N;
+    float* tmp = nullptr;
+    const int nin = c->g.vti ? 7 : 5;
+    HIPCHK(hipMalloc(&tmp, B * nin));
+    const float* srcs[7] = {velp, gama_p, vels, gama_s, rho, eps, delta};
+    int rc = QVTI_OK;
+    for (int k = 0; k < nin && !rc; ++k)
+        if (hipMemcpyAsync(tmp + k * N, srcs[k], B, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            rc = qvti_fail(QVTI_E_DEVICE, "model upload failed");
+    if (!rc) {
+        CoefArgs ca;
+        ca.N = N;
+        ca.vti = c->g.vti;
+        ca.w0 = (double)c->g.w0;
+        ca.dt = (double)c->g.dt;
+        ca.vp = tmp; ca.gp = tmp + N; ca.vs = tmp + 2 * N; ca.gs = tmp + 3 * N; ca.rho = tmp + 4 * N;
+        ca.eps = c->g.vti ? tmp + 5 * N : nullptr;
+        ca.del = c->g.vti ? tmp + 6 * N : nullptr;
+        ca.out = c->coef;
+        const unsigned nb = (unsigned)std::min<long long>((N + kThreads - 1) / kThreads, 65535LL * 4);
+        hipLaunchKernelGGL(k_coefficients, dim3(nb), dim3(kThreads), 0, c->stream, ca);
+        AvgArgs aa;
+        aa.nx = c->g.nx; aa.ny = c->g.ny; aa.nz = c->g.nz;
+        aa.sx = (long long)c->g.ny * c->g.nz; aa.sy = c->g.nz; aa.N = N;
+        aa.dt = c->g.dt;
+        aa.S = c->coef + 5 * N;
+        aa.Sb = c->sbar;
+        aa.rho = tmp + 4 * N;
+        aa.buoy = c->buoy;
+        hipLaunchKernelGGL(k_averages, dim3(nb), dim3(kThreads), 0, c->stream, aa);
+        if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+            rc = qvti_fail(QVTI_E_DEVICE, "model setup kernels failed");
+    }
+    hipFree(tmp);
+    if (!rc) c->has_model = true;
+    return rc;
+}
+
+// ------------------------------------------------------------ running ----
+static int record_event(qvti_ctx* c, int cls)
+{
+    if (!c->timing || c->ev_used >= c->ev_max) return -1;
+    const int k = c->ev_used++;
+    hipEventRecord(c->ev[k], c->stream);
+    c->ev_class[k] = cls;
+    return k;
+}
+
+// Enqueue one time step (14 launches).  With timing on, each launch is
+// bracketed by an event pair.
+static void enqueue_step(qvti_ctx* c)
+{
+    hipStream_t st = c->stream;
+    const dim3 blk(kThreads);
+    const unsigned S = (unsigned)c->nshots;
+    auto timed = [&](int cls, auto&& launch) {
+        const int e0 = c->timing ? record_event(c, cls) : -1;
+        launch();
+        if (e0 >= 0) record_event(c, -1 - cls);
+    };
+    timed(KC_TICK, [&] { hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, c->d_it); });
+    timed(KC_VEL_X, [&] { hipLaunchKernelGGL(k_deriv_strided, dim3(c->g_vx.x, S), blk, 0, st, c->a_vx); });
+    timed(KC_VEL_Y, [&] { hipLaunchKernelGGL(k_deriv_strided, dim3(c->g_vy.x, S), blk, 0, st, c->a_vy); });
+    timed(KC_VEL_Z, [&] { hipLaunchKernelGGL(k_vel_z, dim3(c->g_vz.x, S), blk, 0, st, c->a_vz); });
+    timed(KC_Z_FWD, [&] { hipLaunchKernelGGL(k_zrows, dim3(c->g_zr.x, S), blk, 0, st, c->a_zf); });
+    timed(KC_Y_FWD, [&] { hipLaunchKernelGGL(k_cfft_strided, dim3(c->g_cy.x, S), blk, 0, st, c->a_yf); });
+    timed(KC_XPASS, [&] { hipLaunchKernelGGL(k_xpass, dim3(c->g_xp.x, S), blk, 0, st, c->a_xp); });
+    timed(KC_Y_INV, [&] { hipLaunchKernelGGL(k_cfft_strided, dim3(c->g_cy.x, S), blk, 0, st, c->a_yi); });
+    timed(KC_Z_INV, [&] { hipLaunchKernelGGL(k_zrows, dim3(c->g_zr.x, S), blk, 0, st, c->a_zi); });
+    timed(KC_STR_X, [&] { hipLaunchKernelGGL(k_deriv_strided, dim3(c->g_vx.x, S), blk, 0, st, c->a_sx); });
+    timed(KC_STR_Y, [&] { hipLaunchKernelGGL(k_deriv_strided, dim3(c->g_vy.x, S), blk, 0, st, c->a_sy); });
+    timed(KC_STR_Z, [&] { hipLaunchKernelGGL(k_stress_z, dim3(c->g_sz.x, S), blk, 0, st, c->a_sz); });
+    if (c->total_rec > 0)
+        timed(KC_RECORD, [&] { hipLaunchKernelGGL(k_record, dim3(c->g_rec.x, S), blk, 0, st, c->a_rec); });
+}
+
+static const float* snap_source(qvti_ctx* c)
+{
+    const int f = c->snap_field;
+    if (f < 3) return fld(c->v, f, c);
+    return fld(c->sig, f - 3, c);
+}
+
+// One step plus the snapshot copy of shot 0 when snapshots are enabled.
+static void enqueue_step_with_snap(qvti_ctx* c)
+{
+    enqueue_step(c);
+    if (c->snap_every > 0 && c->d_snap) {
+        const int e0 = c->timing ? record_event(c, KC_SNAP) : -1;
+        hipLaunchKernelGGL(k_snapshot, dim3(256), dim3(kThreads), 0, c->stream, snap_source(c), c->g.nx, c->g.ny,
+                           c->g.nz, c->g.pml, c->snap_every, (const int*)c->d_it, c->d_snap, c->d_snap_cap);
+        if (e0 >= 0) record_event(c, -1 - KC_SNAP);
+    }
+}
+
+static int capture_graph(qvti_ctx* c)
+{
+    if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const bool t = c->timing;
+    c->timing = false;
+    enqueue_step_with_snap(c);
+    c->timing = t;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (e != hipSuccess) return qvti_fail(QVTI_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return qvti_fail(QVTI_E_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+    }
+    return QVTI_OK;
+}
+
+extern "C" int qvti_begin(qvti_ctx* c, int nt, const float* rik, const qvti_shot* shots, int nshots)
+{
+    if (!c || !rik || !shots) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    if (!c->has_model) return qvti_fail(QVTI_E_STATE, "qvti_set_model must be called first");
+    if (nt < 1) return qvti_fail(QVTI_E_ARG, "nt must be >= 1");
+    if (nshots < 1 || nshots > c->S) return qvti_fail(QVTI_E_ARG, "nshots %d not in [1, %d]", nshots, c->S);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const qvti_grid& g = c->g;
+    std::vector<int> kinds(nshots), off(nshots + 1, 0);
+    std::vector<long long> sip(nshots), rip;
+    for (int s = 0; s < nshots; ++s) {
+        const qvti_shot& sh = shots[s];
+        if (sh.kind < QVTI_SRC_FX || sh.kind > QVTI_SRC_EXPLOSIVE)
+            return qvti_fail(QVTI_E_ARG, "shot %d: unknown source kind %d", s, sh.kind);
+        if (sh.sx < 0 || sh.sx >= g.nx || sh.sy < 0 || sh.sy >= g.ny || sh.sz < 0 || sh.sz >= g.nz)
+            return qvti_fail(QVTI_E_ARG, "shot %d: source cell (%d,%d,%d) outside the grid", s, sh.sx, sh.sy, sh.sz);
+        if (sh.nrec < 0 || (sh.nrec > 0 && (!sh.rx || !sh.ry || !sh.rz)))
+            return qvti_fail(QVTI_E_ARG, "shot %d: receiver arrays", s);
+        kinds[s] = sh.kind;
+        sip[s] = ((long long)sh.sx * g.ny + sh.sy) * g.nz + sh.sz;
+        for (int r = 0; r < sh.nrec; ++r) {
+            if (sh.rx[r] < 0 || sh.rx[r] >= g.nx || sh.ry[r] < 0 || sh.ry[r] >= g.ny || sh.rz[r] < 0 || sh.rz[r] >= g.nz)
+                return qvti_fail(QVTI_E_ARG, "shot %d receiver %d outside the grid", s, r);
+            rip.push_back(((long long)sh.rx[r] * g.ny + sh.ry[r]) * g.nz + sh.rz[r]);
+        }
+        off[s + 1] = off[s] + sh.nrec;
+    }
+    // (re)allocate run buffers
+    auto realloc_dev = [&](void** p, size_t bytes) -> int {
+        if (*p) {
+            auto it = std::find(c->allocs.begin(), c->allocs.end(), *p);
+            if (it != c->allocs.end()) c->allocs.erase(it);
+            hipFree(*p);
+            *p = nullptr;
+        }
+        if (bytes == 0) return QVTI_OK;
+        return dalloc(c, p, bytes);
+    };
+    int rc = QVTI_OK;
+    if (c->nt != nt || !c->d_rik) rc = realloc_dev((void**)&c->d_rik, sizeof(float) * nt);
+    const long long rec_need = 3LL * off[nshots] * nt;
+    if (!rc && rec_need > c->rec_cap) {
+        rc = realloc_dev((void**)&c->d_rec, sizeof(float) * rec_need);
+        c->rec_cap = rc ? 0 : rec_need;
+    }
+    if (!rc) rc = realloc_dev((void**)&c->d_rec_ip, sizeof(long long) * std::max<size_t>(rip.size(), 1));
+    if (rc) return rc;
+    c->nt = nt;
+    c->nshots = nshots;
+    c->total_rec = off[nshots];
+    c->steps_done = 0;
+    HIPCHK(hipMemcpyAsync(c->d_rik, rik, sizeof(float) * nt, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_src_kind, kinds.data(), sizeof(int) * nshots, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_src_ip, sip.data(), sizeof(long long) * nshots, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_rec_off, off.data(), sizeof(int) * (nshots + 1), hipMemcpyHostToDevice, c->stream));
+    if (!rip.empty())
+        HIPCHK(hipMemcpyAsync(c->d_rec_ip, rip.data(), sizeof(long long) * rip.size(), hipMemcpyHostToDevice, c->stream));
+    // zero the state that is accumulated rather than overwritten
+    const long long N = c->N, S = c->S;
+    HIPCHK(hipMemsetAsync(c->v, 0, sizeof(float) * 3 * S * N, c->stream));
+    HIPCHK(hipMemsetAsync(c->sig, 0, sizeof(float) * 6 * S * N, c->stream));
+    for (int a = 0; a < 3; ++a) {
+        HIPCHK(hipMemsetAsync(c->phiv[a], 0, sizeof(float) * 3 * S * c->slab[a], c->stream));
+        HIPCHK(hipMemsetAsync(c->phis[a], 0, sizeof(float) * c->nphis[a] * S * c->slab[a], c->stream));
+    }
+    if (c->d_rec) HIPCHK(hipMemsetAsync(c->d_rec, 0, sizeof(float) * rec_need, c->stream));
+    const int minus1 = -1, zero = 0;
+    HIPCHK(hipMemcpyAsync(c->d_it, &minus1, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_probe, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // snapshots
+    if (c->snap_every > 0) {
+        const long long W = (long long)(g.nx - 2 * g.pml) * (g.ny - 2 * g.pml) * (g.nz - 2 * g.pml);
+        const long long nsnap = (nt + c->snap_every - 1) / c->snap_every;
+        const long long need = std::min(W * nsnap, c->snap_cap);
+        if (need > c->d_snap_cap) {
+            rc = realloc_dev((void**)&c->d_snap, sizeof(float) * need);
+            if (rc) return rc;
+            c->d_snap_cap = need;
+        }
+    }
+    // per-run kernel arguments
+    c->a_vz.rik = c->d_rik;
+    c->a_sz.rik = c->d_rik;
+    c->a_rec.rec_ip = c->d_rec_ip;
+    c->a_rec.rx = c->d_rec;
+    c->a_rec.ry = c->d_rec + (long long)c->total_rec * nt;
+    c->a_rec.rz = c->d_rec + 2LL * c->total_rec * nt;
+    c->a_rec.nt = nt;
+    int maxr = 1;
+    for (int s = 0; s < nshots; ++s) maxr = std::max(maxr, shots[s].nrec);
+    c->g_rec = dim3((unsigned)((maxr + kThreads - 1) / kThreads), 1, 1);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->begun = 1;
+    if (c->use_graphs) {
+        rc = capture_graph(c);
+        if (rc) {
+            c->use_graphs = false;  // fall back to direct launches of the same kernels
+            g_err.clear();
+        }
+    }
+    return QVTI_OK;
+}
+
+extern "C" int qvti_step(qvti_ctx* c, int nsteps)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    if (!c->begun) return qvti_fail(QVTI_E_STATE, "qvti_begin must be called first");
+    if (nsteps < 0 || c->steps_done + nsteps > c->nt)
+        return qvti_fail(QVTI_E_ARG, "%d steps requested, %d of %d done", nsteps, c->steps_done, c->nt);
+    HIPCHK(hipSetDevice(c->device));
+    const long long nprobe = 3LL * c->S * c->N;
+    for (int k = 0; k < nsteps; ++k) {
+        if (c->use_graphs && c->gexec && !c->timing) {
+            HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+        } else {
+            enqueue_step_with_snap(c);
+        }
+        c->steps_done++;
+        if (c->probe_every > 0 && (c->steps_done % c->probe_every == 0 || c->steps_done == c->nt)) {
+            const int e0 = c->timing ? record_event(c, KC_PROBE) : -1;
+            hipLaunchKernelGGL(k_probe, dim3(1024), dim3(kThreads), 0, c->stream, (const float*)c->v, nprobe, c->d_probe);
+            if (e0 >= 0) record_event(c, -1 - KC_PROBE);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return QVTI_OK;
+}
+
+static void harvest_events(qvti_ctx* c)
+{
+    for (int k = 0; k + 1 < c->ev_used; ++k) {
+        const int cls = c->ev_class[k];
+        if (cls < 0 || c->ev_class[k + 1] != -1 - cls) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
+            c->kc_ms[cls] += ms;
+            c->kc_launches[cls]++;
+        }
+        ++k;
+    }
+    c->ev_used = 0;
+}
+
+extern "C" int qvti_sync(qvti_ctx* c)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->timing) harvest_events(c);
+    int bits = 0;
+    HIPCHK(hipMemcpy(&bits, c->d_probe, sizeof(int), hipMemcpyDeviceToHost));
+    float lim = c->probe_limit;
+    int lb;
+    memcpy(&lb, &lim, sizeof lb);
+    if (bits > lb) {
+        float m;
+        memcpy(&m, &bits, sizeof m);
+        return qvti_fail(QVTI_E_UNSTABLE, "blow-up probe fired after %d steps: max|v| = %g (NaN/Inf or > %g)",
+                         c->steps_done, (double)m, (double)lim);
+    }
+    return QVTI_OK;
+}
+
+extern "C" int qvti_steps_done(qvti_ctx* c) { return c ? c->steps_done : -1; }
+
+extern "C" int qvti_finish(qvti_ctx* c, float* rec_vx, float* rec_vy, float* rec_vz)
+{
+    if (!c || !c->begun) return qvti_fail(QVTI_E_STATE, "no run in progress");
+    int rc = qvti_sync(c);
+    const long long n = (long long)c->total_rec * c->nt;
+    float* outs[3] = {rec_vx, rec_vy, rec_vz};
+    for (int k = 0; k < 3; ++k)
+        if (outs[k] && n > 0)
+            HIPCHK(hipMemcpy(outs[k], c->d_rec + k * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (c->snap_every > 0 && c->d_snap && c->snap_host) {
+        const qvti_grid& g = c->g;
+        const long long W = (long long)(g.nx - 2 * g.pml) * (g.ny - 2 * g.pml) * (g.nz - 2 * g.pml);
+        const long long nsnap = (c->steps_done - 1) / c->snap_every + 1;
+        const long long cnt = std::min(W * nsnap, c->d_snap_cap);
+        HIPCHK(hipMemcpy(c->snap_host, c->d_snap, sizeof(float) * cnt, hipMemcpyDeviceToHost));
+    }
+    return rc;
+}
+
+extern "C" int qvti_get_state(qvti_ctx* c, int shot, int field, float* out)
+{
+    if (!c || !out) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    if (shot < 0 || shot >= c->S || field < 0 || field > 8) return qvti_fail(QVTI_E_ARG, "shot/field out of range");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const float* base = field < 3 ? fld(c->v, field, c) : fld(c->sig, field - 3, c);
+    HIPCHK(hipMemcpy(out, base + (long long)shot * c->N, sizeof(float) * c->N, hipMemcpyDeviceToHost));
+    return QVTI_OK;
+}
+
+extern "C" int qvti_forward_batch(qvti_ctx* c, int nt, const float* rik, const qvti_shot* shots, int nshots,
+                                  float* rec_vx, float* rec_vy, float* rec_vz)
+{
+    int rc = qvti_begin(c, nt, rik, shots, nshots);
+    if (rc) return rc;
+    // chunks so that a blow-up ends the run early
+    const int chunk = 200;
+    for (int done = 0; done < nt;) {
+        const int k = std::min(chunk, nt - done);
+        rc = qvti_step(c, k);
+        if (rc) return rc;
+        done += k;
+        rc = qvti_sync(c);
+        if (rc) break;
+    }
+    const int rc2 = qvti_finish(c, rec_vx, rec_vy, rec_vz);
+    return rc ? rc : rc2;
+}
+
+extern "C" int qvti_forward(qvti_ctx* c, int nt, const float* rik, const qvti_shot* shot, float* rec_vx,
+                            float* rec_vy, float* rec_vz)
+{
+    return qvti_forward_batch(c, nt, rik, shot, 1, rec_vx, rec_vy, rec_vz);
+}
+
+extern "C" int qvti_forward_acoustic_3D(qvti_ctx* c, int myid, int is, int nt, int ntx, int nty, int ntz, int ntp,
+                                        int nx, int ny, int nz, int pml, float dx, float dy, float dz, float dt,
+                                        float f0, float w0, float velp_max, const float* rik, const float* velp,
+                                        const float* gama_p, const float* vels, const float* gama_s,
+                                        const float* rho, const qvti_ref_source* ss, int rnmax, int rnx_max,
+                                        int rny_max, int dr, float* record, float* record2, float* record3)
+{
+    (void)myid; (void)nx; (void)ny; (void)nz; (void)velp_max; (void)dr;
+    if (!c || !ss) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    const qvti_grid& g = c->g;
+    if (ntx != g.nx || nty != g.ny || ntz != g.nz || (long long)ntp != c->N || pml != g.pml || dx != g.dx ||
+        dy != g.dy || dz != g.dz || dt != g.dt || f0 != g.f0 || w0 != g.w0)
+        return qvti_fail(QVTI_E_ARG, "arguments disagree with the context's grid");
+    const int R = rnx_max * rny_max;
+    if (R < 0 || R > rnmax) return qvti_fail(QVTI_E_ARG, "rnx_max*rny_max exceeds rnmax");
+    int rc = qvti_set_model(c, velp, gama_p, vels, gama_s, rho, nullptr, nullptr);
+    if (rc) return rc;
+    const qvti_ref_source& s = ss[is];
+    std::vector<int> rz(R, s.r_iz);
+    qvti_shot sh;
+    sh.kind = QVTI_SRC_FX;
+    sh.sx = s.s_ix;
+    sh.sy = s.s_iy;
+    sh.sz = s.s_iz + 10;  // cuviscoelas_3D.cu:772
+    sh.nrec = R;
+    sh.rx = s.r_ix;
+    sh.ry = s.r_iy;
+    sh.rz = rz.data();
+    return qvti_forward(c, nt, rik, &sh, record, record2, record3);
+}
+
+extern "C" int qvti_set_snapshot(qvti_ctx* c, int field, int every, float* host_buf, long long capacity)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    if (field < 0 || field > 8 || every < 0 || (every > 0 && (!host_buf || capacity <= 0)))
+        return qvti_fail(QVTI_E_ARG, "bad snapshot arguments");
+    c->snap_field = field;
+    c->snap_every = every;
+    c->snap_host = host_buf;
+    c->snap_cap = capacity;
+    c->begun = 0;  // a new qvti_begin is needed (the step graph changes)
+    return QVTI_OK;
+}
+
+extern "C" int qvti_set_graphs(qvti_ctx* c, int on)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    c->use_graphs = on != 0;
+    return QVTI_OK;
+}
+
+extern "C" int qvti_set_probe(qvti_ctx* c, int every, float limit)
+{
+    if (!c || every < 0 || !(limit > 0)) return qvti_fail(QVTI_E_ARG, "bad probe arguments");
+    c->probe_every = every;
+    c->probe_limit = limit;
+    return QVTI_OK;
+}
+
+extern "C" int qvti_enable_kernel_timing(qvti_ctx* c, int max_steps)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    HIPCHK(hipSetDevice(c->device));
+    for (auto e : c->ev) hipEventDestroy(e);
+    c->ev.clear();
+    c->ev_class.clear();
+    c->ev_used = 0;
+    for (int k = 0; k < KC_COUNT; ++k) { c->kc_ms[k] = 0; c->kc_launches[k] = 0; }
+    c->timing = max_steps > 0;
+    if (!c->timing) return QVTI_OK;
+    c->ev_max = max_steps * 2 * (KC_COUNT + 1);
+    c->ev.resize(c->ev_max);
+    c->ev_class.resize(c->ev_max);
+    for (int k = 0; k < c->ev_max; ++k) HIPCHK(hipEventCreate(&c->ev[k]));
+    return QVTI_OK;
+}
+
+extern "C" int qvti_kernel_times(qvti_ctx* c, int max_classes, char* names, int name_len, long long* launches,
+                                 double* total_ms)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    const int n = std::min(max_classes, (int)KC_COUNT);
+    for (int k = 0; k < n; ++k) {
+        if (names && name_len > 0) {
+            strncpy(names + (long long)k * name_len, kClassNames[k], name_len - 1);
+            names[(long long)k * name_len + name_len - 1] = 0;
+        }
+        if (launches) launches[k] = c->kc_launches[k];
+        if (total_ms) total_ms[k] = c->kc_ms[k];
+    }
+    return n;
+}
+
+// Compulsory HBM bytes of one launch per kernel class: every input field read
+// once, every output written once, over all shots of the launch.
+extern "C" int qvti_kernel_bytes(qvti_ctx* c, int max_classes, double* bytes)
+{
+    if (!c || !bytes) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    const double N = (double)c->N * std::max(1, c->nshots), f = 4.0 * N;
+    const double slab = (double)std::max(1, c->nshots) * 4.0;
+    const double sx = c->slab[0] * slab, sy = c->slab[1] * slab, sz = c->slab[2] * slab;
+    const int ncoef = c->g.vti ? 25 : 15;
+    double b[KC_COUNT] = {};
+    b[KC_TICK] = 8;
+    b[KC_VEL_X] = 6 * f + 3 * 2 * sx;
+    b[KC_VEL_Y] = 9 * f + 3 * 2 * sy;
+    b[KC_VEL_Z] = (3 + 3 + 3 + 3 + 3) * f + 3.0 * 4 * c->N + 3 * 2 * sz;  // sig, acc, v r/w, w, buoy
+    b[KC_Z_FWD] = 6 * f + 6 * f;
+    b[KC_Y_FWD] = 12 * f;
+    b[KC_XPASS] = 6 * f + 12 * f;
+    b[KC_Y_INV] = 24 * f;
+    b[KC_Z_INV] = 12 * f + 9 * f;
+    b[KC_STR_X] = 30 * f;
+    b[KC_STR_Y] = 30 * f;
+    b[KC_STR_Z] = (15 + 30 + 12) * f + ncoef * 4.0 * c->N + 2 * (5 * sx + 5 * sy + 4 * sz);
+    b[KC_RECORD] = 6.0 * 4 * c->total_rec;
+    const int n = std::min(max_classes, (int)KC_COUNT);
+    for (int k = 0; k < n; ++k) bytes[k] = b[k];
+    return n;
+}
+
+// ------------------------------------------------ standalone transforms ---
+extern "C" int qvti_fft3d(int nx, int ny, int nz, int sign, float* host_data)
+{
+    if (!host_data || (sign != 1 && sign != -1)) return qvti_fail(QVTI_E_ARG, "bad arguments");
+    FftPlan px, py, pz;
+    if (!make_plan(nx, &px) || !make_plan(ny, &py) || !make_plan(nz, &pz))
+        return qvti_fail(QVTI_E_UNSUPPORTED, "lengths must be 2/3/5-smooth and <= %d", kMaxN);
+    const long long N = (long long)nx * ny * nz;
+    float2* d = nullptr;
+    float2* tw[3] = {nullptr, nullptr, nullptr};
+    const int dims[3] = {nx, ny, nz};
+    HIPCHK(hipMalloc(&d, sizeof(float2) * N));
+    int rc = QVTI_OK;
+    for (int a = 0; a < 3 && !rc; ++a) {
+        std::vector<float2> t(dims[a]);
+        for (int i = 0; i < dims[a]; ++i) {
+            const double ang = -2.0 * M_PI * i / dims[a];
+            t[i] = make_float2((float)cos(ang), (float)sin(ang));
+        }
+        if (hipMalloc(&tw[a], sizeof(float2) * dims[a]) != hipSuccess ||
+            hipMemcpy(tw[a], t.data(), sizeof(float2) * dims[a], hipMemcpyHostToDevice) != hipSuccess)
+            rc = qvti_fail(QVTI_E_DEVICE, "table upload failed");
+    }
+    if (!rc && hipMemcpy(d, host_data, sizeof(float2) * N, hipMemcpyHostToDevice) != hipSuccess)
+        rc = qvti_fail(QVTI_E_DEVICE, "H2D failed");
+    if (!rc) {
+        const long long nrows = (long long)nx * ny;
+        ZRowsArgs z{};
+        z.nz = nz; z.Pc = pick_pc(kTileMax, nz, (int)std::min<long long>(nrows, 64)); z.sign = (float)sign;
+        z.nrows = nrows; z.shot_stride = N; z.plan = pz; z.tw = tw[2]; z.mode = 2; z.nfields = 1;
+        z.cin[0] = d; z.cout[0] = d;
+        hipLaunchKernelGGL(k_zrows, dim3((unsigned)((nrows + z.Pc - 1) / z.Pc), 1), dim3(kThreads), 0, 0, z);
+        for (int axis = 1; axis >= 0; --axis) {
+            CStridedArgs y{};
+            y.nz = nz; y.sx = (long long)ny * nz; y.sy = nz; y.axis = axis; y.n = dims[axis];
+            y.Pc = pick_pc(kTileMax, y.n, nz); y.ztiles = (nz + y.Pc - 1) / y.Pc; y.sign = (float)sign;
+            y.shot_stride = N; y.plan = axis ? py : px; y.tw = tw[axis]; y.nfields = 1; y.f[0] = d;
+            const int other = axis ? nx : ny;
+            hipLaunchKernelGGL(k_cfft_strided, dim3(other * y.ztiles, 1), dim3(kThreads), 0, 0, y);
+        }
+        if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
+            rc = qvti_fail(QVTI_E_DEVICE, "fft kernels failed");
+    }
+    if (!rc && hipMemcpy(host_data, d, sizeof(float2) * N, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = qvti_fail(QVTI_E_DEVICE, "D2H failed");
+    hipFree(d);
+    for (auto p : tw) if (p) hipFree(p);
+    return rc;
+}
+
+extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym, const float* in, float* out)
+{
+    if (!in || !out || axis < 0 || axis > 2 || sym < 0 || sym > 1 || !(dd > 0))
+        return qvti_fail(QVTI_E_ARG, "bad arguments");
+    const int dims[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a)
+        if (!supported_n(dims[a])) return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d unsupported", a, dims[a]);
+    const int n = dims[axis];
+    FftPlan pl;
+    make_plan(n, &pl);
+    std::vector<float2> tw(n), dp, dm;
+    std::vector<float> kf;
+    for (int i = 0; i < n; ++i) {
+        const double ang = -2.0 * M_PI * i / n;
+        tw[i] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    axis_symbols(n, dd, dp, dm, kf);
+    const long long N = (long long)nx * ny * nz;
+    float *din = nullptr, *dout = nullptr;
+    float2 *dtw = nullptr, *dsym = nullptr;
+    int rc = QVTI_OK;
+    if (hipMalloc(&din, sizeof(float) * N) != hipSuccess || hipMalloc(&dout, sizeof(float) * N) != hipSuccess ||
+        hipMalloc(&dtw, sizeof(float2) * n) != hipSuccess || hipMalloc(&dsym, sizeof(float2) * n) != hipSuccess)
+        rc = qvti_fail(QVTI_E_NOMEM, "allocation failed");
+    if (!rc && (hipMemcpy(din, in, sizeof(float) * N, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dtw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(dsym, (sym ? dm : dp).data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess))
+        rc = qvti_fail(QVTI_E_DEVICE, "H2D failed");
+    if (!rc) {
+        if (axis < 2) {
+            StridedArgs a{};
+            a.nx = nx; a.ny = ny; a.nz = nz; a.pml = 0;
+            a.sx = (long long)ny * nz; a.sy = nz; a.axis = axis; a.n = n;
+            a.Pc = pick_pc(kTileMax, n, nz / 2); a.ztiles = (nz + 2 * a.Pc - 1) / (2 * a.Pc);
+            a.mode = MODE_STORE; a.dt = 0.f; a.shot_stride = N; a.slab_stride = 0;
+            a.plan = pl; a.tw = dtw; a.sym[0] = dsym; a.sym[1] = dsym;
+            a.njobs = 1;
+            a.job[0].in = din; a.job[0].out = dout; a.job[0].acc = nullptr; a.job[0].phi = nullptr;
+            a.job[0].sym = 0; a.job[0].half = 0;
+            const int other = axis == 0 ? ny : nx;
+            hipLaunchKernelGGL(k_deriv_strided, dim3(other * a.ztiles, 1), dim3(kThreads), 0, 0, a);
+        } else {
+            RowDerivArgs r{};
+            const long long nrows = (long long)nx * ny;
+            r.nz = nz; r.Pc = pick_pc(kTileMax, nz, (int)std::min<long long>((nrows + 1) / 2, 64));
+            r.nrows = nrows; r.plan = pl; r.tw = dtw; r.sym = dsym; r.in = din; r.out = dout;
+            hipLaunchKernelGGL(k_deriv_rows, dim3((unsigned)((nrows + 2 * r.Pc - 1) / (2 * r.Pc))), dim3(kThreads), 0, 0, r);
+        }
+        if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
+            rc = qvti_fail(QVTI_E_DEVICE, "derivative kernel failed");
+    }
+    if (!rc && hipMemcpy(out, dout, sizeof(float) * N, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = qvti_fail(QVTI_E_DEVICE, "D2H failed");
+    hipFree(din); hipFree(dout); hipFree(dtw); hipFree(dsym);
+    return rc;
+}

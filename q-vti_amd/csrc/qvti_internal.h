@@ -1,0 +1,5 @@
+// qvti_internal.h -- declarations shared by the engine's translation units.
+#pragma once
+
+// Record a formatted message as this thread's last error and return `code`.
+int qvti_fail(int code, const char* fmt, ...);
