@@ -126,9 +126,121 @@ static void FN(fft_pencil)(const FN(plan_t) *p, int sign, REAL *xr, REAL *xi, RE
     }
 }
 
-/* 3-D transform of interleaved complex data [n0][n1][n2], axis by axis.
- * Reference: cufftExecC2C on a cufftPlan3d(ntx,nty,ntz) plan
- * (cuviscoelas_3D.cu:1703-1704), unnormalised. */
+/* Blocked variant: QO_B pencils side by side, element (i, b) at i*QO_B + b,
+ * so the inner loop over b runs with one twiddle set (vectorisable).  Same
+ * Stockham factorisation and twiddles as fft_pencil. */
+#ifndef QO_B
+#define QO_B 16
+#endif
+static void FN(fft_block)(const FN(plan_t) *p, int sign, REAL *xr, REAL *xi, REAL *yr, REAL *yi)
+{
+    const int n = p->n, B = QO_B;
+    int Ns = 1;
+    REAL *sr = xr, *si = xi, *dr = yr, *di = yi;
+    for (int f = 0; f < p->nf; f++) {
+        const int R = p->fac[f], m = n / R, ts = n / (Ns * R);
+        for (int j = 0; j < m; j++) {
+            const int k = j % Ns;
+            const int d = (j - k) * R + k;
+            REAL wr[QO_MAXR], wi[QO_MAXR];
+            wr[0] = 1; wi[0] = 0;
+            for (int r = 1; r < R; r++) {
+                const int t = k * r * ts;
+                wr[r] = p->twr[t];
+                wi[r] = (sign < 0) ? p->twi[t] : -p->twi[t];
+            }
+            if (R == 2) {
+                for (int b = 0; b < B; b++) {
+                    REAL ar = sr[j * B + b], ai = si[j * B + b];
+                    REAL br0 = sr[(j + m) * B + b], bi0 = si[(j + m) * B + b];
+                    REAL br = br0 * wr[1] - bi0 * wi[1], bi = br0 * wi[1] + bi0 * wr[1];
+                    dr[d * B + b] = ar + br; di[d * B + b] = ai + bi;
+                    dr[(d + Ns) * B + b] = ar - br; di[(d + Ns) * B + b] = ai - bi;
+                }
+            } else if (R == 4) {
+                const REAL sg = (REAL)sign;
+                for (int b = 0; b < B; b++) {
+                    REAL v0r = sr[j * B + b], v0i = si[j * B + b];
+                    REAL u, q;
+                    u = sr[(j + m) * B + b]; q = si[(j + m) * B + b];
+                    REAL v1r = u * wr[1] - q * wi[1], v1i = u * wi[1] + q * wr[1];
+                    u = sr[(j + 2 * m) * B + b]; q = si[(j + 2 * m) * B + b];
+                    REAL v2r = u * wr[2] - q * wi[2], v2i = u * wi[2] + q * wr[2];
+                    u = sr[(j + 3 * m) * B + b]; q = si[(j + 3 * m) * B + b];
+                    REAL v3r = u * wr[3] - q * wi[3], v3i = u * wi[3] + q * wr[3];
+                    REAL t0r = v0r + v2r, t0i = v0i + v2i, t1r = v0r - v2r, t1i = v0i - v2i;
+                    REAL t2r = v1r + v3r, t2i = v1i + v3i, t3r = v1r - v3r, t3i = v1i - v3i;
+                    REAL ur = -sg * t3i, ui = sg * t3r;
+                    dr[d * B + b] = t0r + t2r; di[d * B + b] = t0i + t2i;
+                    dr[(d + 2 * Ns) * B + b] = t0r - t2r; di[(d + 2 * Ns) * B + b] = t0i - t2i;
+                    dr[(d + Ns) * B + b] = t1r + ur; di[(d + Ns) * B + b] = t1i + ui;
+                    dr[(d + 3 * Ns) * B + b] = t1r - ur; di[(d + 3 * Ns) * B + b] = t1i - ui;
+                }
+            } else if (R == 3) {
+                const REAL h = (REAL)(sign * 0.86602540378443864676);
+                for (int b = 0; b < B; b++) {
+                    REAL v0r = sr[j * B + b], v0i = si[j * B + b], u, q;
+                    u = sr[(j + m) * B + b]; q = si[(j + m) * B + b];
+                    REAL v1r = u * wr[1] - q * wi[1], v1i = u * wi[1] + q * wr[1];
+                    u = sr[(j + 2 * m) * B + b]; q = si[(j + 2 * m) * B + b];
+                    REAL v2r = u * wr[2] - q * wi[2], v2i = u * wi[2] + q * wr[2];
+                    REAL t1r = v1r + v2r, t1i = v1i + v2i;
+                    REAL t2r = v0r - (REAL)0.5 * t1r, t2i = v0i - (REAL)0.5 * t1i;
+                    REAL t3r = -h * (v1i - v2i), t3i = h * (v1r - v2r);
+                    dr[d * B + b] = v0r + t1r; di[d * B + b] = v0i + t1i;
+                    dr[(d + Ns) * B + b] = t2r + t3r; di[(d + Ns) * B + b] = t2i + t3i;
+                    dr[(d + 2 * Ns) * B + b] = t2r - t3r; di[(d + 2 * Ns) * B + b] = t2i - t3i;
+                }
+            } else if (R == 5) {
+                const REAL c1 = (REAL)0.30901699437494742410, c2 = (REAL)-0.80901699437494742410;
+                const REAL s1 = (REAL)(sign * 0.95105651629515357212), s2 = (REAL)(sign * 0.58778525229247312917);
+                for (int b = 0; b < B; b++) {
+                    REAL xr5[5], xi5[5];
+                    for (int r = 0; r < 5; r++) {
+                        REAL u = sr[(j + r * m) * B + b], q = si[(j + r * m) * B + b];
+                        xr5[r] = u * wr[r] - q * wi[r];
+                        xi5[r] = u * wi[r] + q * wr[r];
+                    }
+                    REAL t1r = xr5[1] + xr5[4], t1i = xi5[1] + xi5[4], t2r = xr5[2] + xr5[3], t2i = xi5[2] + xi5[3];
+                    REAL t3r = xr5[1] - xr5[4], t3i = xi5[1] - xi5[4], t4r = xr5[2] - xr5[3], t4i = xi5[2] - xi5[3];
+                    REAL a1r = xr5[0] + c1 * t1r + c2 * t2r, a1i = xi5[0] + c1 * t1i + c2 * t2i;
+                    REAL a2r = xr5[0] + c2 * t1r + c1 * t2r, a2i = xi5[0] + c2 * t1i + c1 * t2i;
+                    REAL e1r = s1 * t3r + s2 * t4r, e1i = s1 * t3i + s2 * t4i;
+                    REAL e2r = s2 * t3r - s1 * t4r, e2i = s2 * t3i - s1 * t4i;
+                    /* multiply by i: (re, im) -> (-im, re) */
+                    dr[d * B + b] = xr5[0] + t1r + t2r; di[d * B + b] = xi5[0] + t1i + t2i;
+                    dr[(d + Ns) * B + b] = a1r - e1i; di[(d + Ns) * B + b] = a1i + e1r;
+                    dr[(d + 4 * Ns) * B + b] = a1r + e1i; di[(d + 4 * Ns) * B + b] = a1i - e1r;
+                    dr[(d + 2 * Ns) * B + b] = a2r - e2i; di[(d + 2 * Ns) * B + b] = a2i + e2r;
+                    dr[(d + 3 * Ns) * B + b] = a2r + e2i; di[(d + 3 * Ns) * B + b] = a2i - e2r;
+                }
+            } else {
+                for (int b = 0; b < B; b++) {
+                    REAL vr[QO_MAXR], vi[QO_MAXR];
+                    for (int r = 0; r < R; r++) {
+                        REAL u = sr[(j + r * m) * B + b], q = si[(j + r * m) * B + b];
+                        vr[r] = u * wr[r] - q * wi[r];
+                        vi[r] = u * wi[r] + q * wr[r];
+                    }
+                    FN(dft_small)(R, sign, vr, vi, p);
+                    for (int r = 0; r < R; r++) { dr[(d + r * Ns) * B + b] = vr[r]; di[(d + r * Ns) * B + b] = vi[r]; }
+                }
+            }
+        }
+        REAL *t;
+        t = sr; sr = dr; dr = t;
+        t = si; si = di; di = t;
+        Ns *= R;
+    }
+    if (sr != xr) {
+        memcpy(xr, sr, sizeof(REAL) * n * B);
+        memcpy(xi, si, sizeof(REAL) * n * B);
+    }
+}
+
+/* 3-D transform of interleaved complex data [n0][n1][n2], axis by axis,
+ * QO_B pencils at a time.  Reference: cufftExecC2C on a
+ * cufftPlan3d(ntx,nty,ntz) plan (cuviscoelas_3D.cu:1703-1704), unnormalised. */
 typedef struct {
     int n[3];
     FN(plan_t) p[3];
@@ -149,30 +261,54 @@ static void FN(plan3_free)(FN(plan3_t) *q)
 
 static void FN(fft3d_exec)(const FN(plan3_t) *q, int sign, REAL *data)
 {
-    int n0 = q->n[0], n1 = q->n[1], n2 = q->n[2];
+    const int n0 = q->n[0], n1 = q->n[1], n2 = q->n[2], B = QO_B;
     for (int axis = 2; axis >= 0; axis--) {
-        int n = q->n[axis];
-        long stride = (axis == 2) ? 1 : (axis == 1 ? n2 : (long)n1 * n2);
-        long npencil = (long)n0 * n1 * n2 / n;
+        const int n = q->n[axis];
+        /* pencils are grouped B at a time: along z, B consecutive rows; along
+         * y and x, B consecutive z-columns of one plane */
+        long nblk;
+        if (axis == 2) nblk = ((long)n0 * n1 + B - 1) / B;
+        else if (axis == 1) nblk = (long)n0 * ((n2 + B - 1) / B);
+        else nblk = (long)n1 * ((n2 + B - 1) / B);
 #pragma omp parallel
         {
-            REAL *buf = (REAL *)malloc(sizeof(REAL) * 4 * n);
-            REAL *xr = buf, *xi = buf + n, *yr = buf + 2 * n, *yi = buf + 3 * n;
+            REAL *buf = (REAL *)malloc(sizeof(REAL) * 4 * (size_t)n * B);
+            REAL *xr = buf, *xi = buf + (size_t)n * B, *yr = buf + 2 * (size_t)n * B, *yi = buf + 3 * (size_t)n * B;
 #pragma omp for schedule(static)
-            for (long pc = 0; pc < npencil; pc++) {
-                long base;
-                if (axis == 2) base = pc * n2;
-                else if (axis == 1) base = (pc / n2) * (long)n1 * n2 + (pc % n2);
-                else base = pc;
-                for (int i = 0; i < n; i++) {
-                    xr[i] = data[2 * (base + i * stride)];
-                    xi[i] = data[2 * (base + i * stride) + 1];
+            for (long blk = 0; blk < nblk; blk++) {
+                long base, stride, pstep;
+                int cnt;
+                if (axis == 2) {
+                    const long r0 = blk * B, nr = (long)n0 * n1;
+                    cnt = (int)(nr - r0 < B ? nr - r0 : B);
+                    base = r0 * n2; stride = 1; pstep = n2;
+                } else {
+                    const int zb = (n2 + B - 1) / B;
+                    const long other = blk / zb;
+                    const int z0 = (int)(blk % zb) * B;
+                    cnt = n2 - z0 < B ? n2 - z0 : B;
+                    base = (axis == 1 ? other * (long)n1 * n2 : other * (long)n2) + z0;
+                    stride = axis == 1 ? n2 : (long)n1 * n2;
+                    pstep = 1;
                 }
-                FN(fft_pencil)(&q->p[axis], sign, xr, xi, yr, yi);
-                for (int i = 0; i < n; i++) {
-                    data[2 * (base + i * stride)] = xr[i];
-                    data[2 * (base + i * stride) + 1] = xi[i];
-                }
+                for (int i = 0; i < n; i++)
+                    for (int b = 0; b < B; b++) {
+                        if (b < cnt) {
+                            const long g = 2 * (base + i * stride + b * pstep);
+                            xr[i * B + b] = data[g];
+                            xi[i * B + b] = data[g + 1];
+                        } else {
+                            xr[i * B + b] = 0;
+                            xi[i * B + b] = 0;
+                        }
+                    }
+                FN(fft_block)(&q->p[axis], sign, xr, xi, yr, yi);
+                for (int i = 0; i < n; i++)
+                    for (int b = 0; b < cnt; b++) {
+                        const long g = 2 * (base + i * stride + b * pstep);
+                        data[g] = xr[i * B + b];
+                        data[g + 1] = xi[i * B + b];
+                    }
             }
             free(buf);
         }
