@@ -15,8 +15,9 @@
 #include <vector>
 
 #include "qvti.h"
+#include "qvti_aux.h"
 #include "qvti_internal.h"
-#include "qvti_kernels.h"
+#include "qvti_pencils.h"
 
 using namespace qvti;
 
@@ -45,21 +46,41 @@ extern "C" const char* qvti_last_error(void) { return g_err.c_str(); }
                              __LINE__);                                                    \
     } while (0)
 
-// ------------------------------------------------------------ plans ------
-// Radix order: 4s first, then 2, 3, 5 (same factorisation the oracle uses).
-static bool make_plan(int n, FftPlan* p)
+// ------------------------------------------------------------ lengths ----
+constexpr int kMaxN = 512;
+
+// Run f(L2<N1,N2>{}) for the supported axis length n; false if unsupported.
+template <class F>
+static bool dispatch_n(int n, F&& f)
 {
-    if (n < 2 || n > kMaxN) return false;
-    int m = n, np = 0;
-    while (m % 4 == 0) { p->radix[np++] = 4; m /= 4; }
-    while (m % 2 == 0) { p->radix[np++] = 2; m /= 2; }
-    while (m % 3 == 0) { p->radix[np++] = 3; m /= 3; }
-    while (m % 5 == 0) { p->radix[np++] = 5; m /= 5; }
-    if (m != 1 || np > kMaxPasses) return false;
-    p->n = n;
-    p->npass = np;
-    return true;
+    switch (n) {
+    case 16: f(L2<4, 4>{}); return true;
+    case 20: f(L2<4, 5>{}); return true;
+    case 24: f(L2<4, 6>{}); return true;
+    case 32: f(L2<4, 8>{}); return true;
+    case 40: f(L2<5, 8>{}); return true;
+    case 48: f(L2<6, 8>{}); return true;
+    case 64: f(L2<8, 8>{}); return true;
+    case 96: f(L2<8, 12>{}); return true;
+    case 100: f(L2<10, 10>{}); return true;
+    case 128: f(L2<8, 16>{}); return true;
+    case 160: f(L2<10, 16>{}); return true;
+    case 200: f(L2<10, 20>{}); return true;
+    case 256: f(L2<16, 16>{}); return true;
+    case 400: f(L2<20, 20>{}); return true;
+    default: return false;
+    }
 }
+static const char* kSupportedList = "16 20 24 32 40 48 64 96 100 128 160 200 256 400";
+
+static int tile_pencils(int n)
+{
+    int P = 0;
+    dispatch_n(n, [&](auto g) { P = decltype(g)::P; });
+    return P;
+}
+
+static bool supported_n(int n) { return tile_pencils(n) > 0; }
 
 // The reference's truncated pi (cuviscoelas_3D.cu:15).
 static const double kRefPi = 3.1415926;
@@ -122,21 +143,19 @@ static void pml_profile(int n, int pml, float d, float f0, std::vector<float>& a
     }
 }
 
-static bool supported_n(int n) { FftPlan p; return n >= 8 && n % 2 == 0 && make_plan(n, &p); }
 
 // ------------------------------------------------------------ context ----
 namespace {
 enum KClass {
-    KC_TICK = 0, KC_VEL_X, KC_VEL_Y, KC_VEL_Z, KC_Z_FWD, KC_Y_FWD, KC_XPASS, KC_Y_INV, KC_Z_INV,
+    KC_TICK = 0, KC_VEL_X, KC_VEL_Y, KC_VEL_Z, KC_Y_FWD, KC_XPASS, KC_Y_INV, KC_Z_INV,
     KC_STR_X, KC_STR_Y, KC_STR_Z, KC_RECORD, KC_SNAP, KC_PROBE, KC_COUNT
 };
-const char* kClassNames[KC_COUNT] = {"tick", "vel_x", "vel_y", "vel_z", "z_fwd", "y_fwd", "xpass", "y_inv",
+const char* kClassNames[KC_COUNT] = {"tick", "vel_x", "vel_y", "vel_z_fwd", "y_fwd", "xpass", "y_inv",
                                      "z_inv", "str_x", "str_y", "str_z", "record", "snapshot", "probe"};
 
 struct AxisDev {
     int n;
     float d;
-    FftPlan plan;
     float2 *tw = nullptr, *dp = nullptr, *dm = nullptr;
     float *kf = nullptr, *a = nullptr, *b = nullptr, *ah = nullptr, *bh = nullptr;
 };
@@ -192,12 +211,13 @@ struct qvti_ctx {
     // launch configuration
     StridedArgs a_vx{}, a_vy{}, a_sx{}, a_sy{};
     VelZArgs a_vz{};
-    ZRowsArgs a_zf{}, a_zi{};
+    ZRowsArgs a_zi{};
     CStridedArgs a_yf{}, a_yi{};
     XPassArgs a_xp{};
     StressZArgs a_sz{};
     RecArgs a_rec{};
     dim3 g_vx, g_vy, g_vz, g_zr, g_cy, g_xp, g_sz, g_rec;
+    int P[3] = {0, 0, 0};
     long long bytes = 0;
     std::vector<void*> allocs;
 };
@@ -263,15 +283,6 @@ extern "C" int qvti_device_count(int* n)
 
 extern "C" long long qvti_device_bytes(qvti_ctx* c) { return c ? c->bytes : -1; }
 
-static int pick_pc(int cap, int n, int limit)
-{
-    int p = cap / n;
-    if (p > 64) p = 64;
-    if (p > limit) p = limit;
-    if (p < 1) p = 1;
-    return p;
-}
-
 static int build_launch_config(qvti_ctx* c);
 
 extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, int device)
@@ -283,7 +294,7 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
     for (int a = 0; a < 3; ++a)
         if (!supported_n(dims[a]))
             return qvti_fail(QVTI_E_UNSUPPORTED,
-                             "axis %d length %d: the engine needs even 2/3/5-smooth lengths in [8, %d]", a, dims[a], kMaxN);
+                             "axis %d length %d: supported axis lengths are %s", a, dims[a], kSupportedList);
     if (g->pml < 2 || 2 * g->pml > g->nx || 2 * g->pml > g->ny || 2 * g->pml > g->nz)
         return qvti_fail(QVTI_E_ARG, "pml %d invalid for grid %dx%dx%d", g->pml, g->nx, g->ny, g->nz);
     if (!(g->dx > 0 && g->dy > 0 && g->dz > 0 && g->dt > 0 && g->w0 > 0 && g->f0 > 0))
@@ -309,7 +320,6 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
         AxisDev& A = c->ax[a];
         A.n = dims[a];
         A.d = ds[a];
-        make_plan(A.n, &A.plan);
         std::vector<float2> tw(A.n), dp, dm;
         for (int t = 0; t < A.n; ++t) {
             const double ang = -2.0 * M_PI * t / A.n;
@@ -382,30 +392,26 @@ static int build_launch_config(qvti_ctx* c)
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long sx = (long long)ny * nz, sy = nz, N = c->N;
     const AxisDev &X = c->ax[0], &Y = c->ax[1], &Z = c->ax[2];
-    const int nsets = g.vti ? 4 : 2;
-    (void)nsets;
+    for (int a = 0; a < 3; ++a) c->P[a] = tile_pencils(c->ax[a].n);
+    const int Px = c->P[0], Py = c->P[1], Pz = c->P[2];
 
     float* acc[3] = {fld(c->bufA, 0, c), fld(c->bufA, 1, c), fld(c->bufA, 2, c)};
     float* vv[3] = {fld(c->v, 0, c), fld(c->v, 1, c), fld(c->v, 2, c)};
-    float* ww[3] = {fld(c->dv, 0, c), fld(c->dv, 1, c), fld(c->dv, 2, c)};
+    float* dvv[3] = {fld(c->dv, 0, c), fld(c->dv, 1, c), fld(c->dv, 2, c)};
     // sig order: pxx pyy pzz pxy pxz pyz
     float* sg[6];
     for (int k = 0; k < 6; ++k) sg[k] = fld(c->sig, k, c);
     float *pxx = sg[0], *pyy = sg[1], *pzz = sg[2], *pxy = sg[3], *pxz = sg[4], *pyz = sg[5];
 
-    // ---- velocity x: pxx (D-, half) -> acc_x, pxy (D+) -> acc_y, pxz (D+) -> acc_z
-    auto strided_common = [&](StridedArgs& a, int axis) {
+    auto strided_common = [&](StridedArgs& a, int axis, int P) {
         const AxisDev& A = c->ax[axis];
         a.nx = nx; a.ny = ny; a.nz = nz; a.pml = g.pml;
         a.sx = sx; a.sy = sy;
         a.axis = axis;
-        a.n = A.n;
-        a.Pc = pick_pc(kTileMax, A.n, nz / 2);
-        a.ztiles = (nz + 2 * a.Pc - 1) / (2 * a.Pc);
+        a.ztiles = (nz + 2 * P - 1) / (2 * P);
         a.dt = g.dt;
         a.shot_stride = N;
         a.slab_stride = c->slab[axis];
-        a.plan = A.plan;
         a.tw = A.tw;
         a.sym[0] = A.dp;
         a.sym[1] = A.dm;
@@ -416,19 +422,20 @@ static int build_launch_config(qvti_ctx* c)
         j.in = in; j.out = out; j.acc = accp; j.phi = phi; j.sym = sym; j.half = half;
         return j;
     };
+    // ---- velocity x: pxx (D-, half) -> acc_x, pxy (D+) -> acc_y, pxz (D+) -> acc_z
     StridedArgs& vx = c->a_vx;
-    strided_common(vx, 0);
+    strided_common(vx, 0, Px);
     vx.mode = MODE_VEL_FIRST;
     vx.njobs = 3;
     vx.job[0] = job(pxx, acc[0], nullptr, nullptr, 1, 1);
     vx.job[1] = job(pxy, acc[1], nullptr, nullptr, 0, 0);
     vx.job[2] = job(pxz, acc[2], nullptr, nullptr, 0, 0);
-    // slab arrays are [field][shot][slab]
-    for (int k = 0; k < 3; ++k) vx.job[k].phi = c->phiv[0] + (long long)k * c->S * c->slab[0];
+    for (int k = 0; k < 3; ++k) vx.job[k].phi = c->phiv[0] + (long long)k * c->S * c->slab[0];  // [field][shot][slab]
     c->g_vx = dim3(ny * vx.ztiles, 1, 1);
 
+    // ---- velocity y: pxy (D+) -> acc_x, pyy (D-, half) -> acc_y, pyz (D+) -> acc_z
     StridedArgs& vy = c->a_vy;
-    strided_common(vy, 1);
+    strided_common(vy, 1, Py);
     vy.mode = MODE_VEL_ACC;
     vy.njobs = 3;
     vy.job[0] = job(pxy, acc[0], acc[0], nullptr, 0, 0);
@@ -437,33 +444,8 @@ static int build_launch_config(qvti_ctx* c)
     for (int k = 0; k < 3; ++k) vy.job[k].phi = c->phiv[1] + (long long)k * c->S * c->slab[1];
     c->g_vy = dim3(nx * vy.ztiles, 1, 1);
 
-    // ---- velocity z + update: pxz (D+) -> vx, pyz (D+) -> vy, pzz (D-, half) -> vz
-    VelZArgs& vz = c->a_vz;
-    vz.nx = nx; vz.ny = ny; vz.nz = nz; vz.pml = g.pml;
-    vz.Pc = pick_pc(kTileMax, nz, (int)((c->nrows + 1) / 2));
-    vz.nrows = c->nrows;
-    vz.dt = g.dt;
-    vz.shot_stride = N;
-    vz.slab_stride = c->slab[2];
-    vz.plan = Z.plan; vz.tw = Z.tw; vz.sym[0] = Z.dp; vz.sym[1] = Z.dm;
-    vz.pa = Z.a; vz.pb = Z.b; vz.pah = Z.ah; vz.pbh = Z.bh;
-    vz.sig[0] = pxz; vz.sig[1] = pyz; vz.sig[2] = pzz;
-    vz.symsel[0] = 0; vz.symsel[1] = 0; vz.symsel[2] = 1;
-    vz.half[0] = 0; vz.half[1] = 0; vz.half[2] = 1;
-    for (int k = 0; k < 3; ++k) {
-        vz.acc[k] = acc[k];
-        vz.v[k] = vv[k];
-        vz.dv[k] = ww[k];
-        vz.buoy[k] = c->buoy + (long long)k * N;
-        vz.phi[k] = c->phiv[2] + (long long)k * c->S * c->slab[2];
-    }
-    vz.src_kind = c->d_src_kind;
-    vz.src_ip = c->d_src_ip;
-    vz.rik = nullptr;  // set at begin
-    vz.it = c->d_it;
-    c->g_vz = dim3((unsigned)((c->nrows + 2 * vz.Pc - 1) / (2 * vz.Pc)), 1, 1);
-
-    // ---- 3-D pipeline.  P_c = v_c + i dv_c (bufB as complex), Q1_c, Q3_c (bufA as complex)
+    // ---- 3-D transform buffers: P_c = FFT(v_c + i dv_c) (bufB as complex),
+    // Q1_c, Q3_c (bufA as complex)
     float2* P[3];
     float2* Q1[3];
     float2* Q3[3];
@@ -472,45 +454,55 @@ static int build_launch_config(qvti_ctx* c)
         Q1[k] = reinterpret_cast<float2*>(fld(c->bufA, 2 * k, c));
         Q3[k] = reinterpret_cast<float2*>(fld(c->bufA, 6 + 2 * k, c));
     }
+    // ---- velocity z + update + forward z-FFT: pxz (D+) -> vx, pyz (D+) -> vy, pzz (D-, half) -> vz
+    VelZArgs& vz = c->a_vz;
+    vz.nx = nx; vz.ny = ny; vz.nz = nz; vz.pml = g.pml;
+    vz.nrows = c->nrows;
+    vz.dt = g.dt;
+    vz.shot_stride = N;
+    vz.slab_stride = c->slab[2];
+    vz.tw = Z.tw; vz.symp = Z.dp; vz.symm = Z.dm;
+    vz.pa = Z.a; vz.pb = Z.b; vz.pah = Z.ah; vz.pbh = Z.bh;
+    vz.pxz = pxz; vz.pyz = pyz; vz.pzz = pzz;
+    for (int k = 0; k < 3; ++k) {
+        vz.acc[k] = acc[k];
+        vz.v[k] = vv[k];
+        vz.dv[k] = dvv[k];
+        vz.P[k] = P[k];
+        vz.buoy[k] = c->buoy + (long long)k * N;
+        vz.phi[k] = c->phiv[2] + (long long)k * c->S * c->slab[2];
+    }
+    vz.src_kind = c->d_src_kind;
+    vz.src_ip = c->d_src_ip;
+    vz.rik = nullptr;  // set at begin
+    vz.it = c->d_it;
+    c->g_vz = dim3((unsigned)((c->nrows + Pz - 1) / Pz), 1, 1);
+    c->g_zr = c->g_vz;
+
     // H: [A1 v x,y,z][A1 dv x,y,z][A3 v x,y,z]
     float* H1v[3];
-    float* H1w[3];
+    float* H1dv[3];
     float* H3v[3];
     for (int k = 0; k < 3; ++k) {
         H1v[k] = fld(c->H, k, c);
-        H1w[k] = fld(c->H, 3 + k, c);
+        H1dv[k] = fld(c->H, 3 + k, c);
         H3v[k] = fld(c->H, 6 + k, c);
     }
-    ZRowsArgs& zf = c->a_zf;
-    zf.nz = nz;
-    zf.Pc = pick_pc(kTileMax, nz, (int)c->nrows);
-    zf.sign = -1.f;
-    zf.nrows = c->nrows;
-    zf.shot_stride = N;
-    zf.plan = Z.plan; zf.tw = Z.tw;
-    zf.mode = 0;
-    zf.nfields = 3;
-    for (int k = 0; k < 3; ++k) { zf.re_in[k] = vv[k]; zf.im_in[k] = ww[k]; zf.cout[k] = P[k]; }
-    c->g_zr = dim3((unsigned)((c->nrows + zf.Pc - 1) / zf.Pc), 1, 1);
-
     CStridedArgs& yf = c->a_yf;
     yf.nz = nz; yf.sx = sx; yf.sy = sy;
-    yf.axis = 1; yf.n = ny;
-    yf.Pc = pick_pc(kTileMax, ny, nz);
-    yf.ztiles = (nz + yf.Pc - 1) / yf.Pc;
-    yf.sign = -1.f;
+    yf.axis = 1;
+    yf.ztiles = (nz + Py - 1) / Py;
     yf.shot_stride = N;
-    yf.plan = Y.plan; yf.tw = Y.tw;
+    yf.tw = Y.tw;
     yf.nfields = 3;
     for (int k = 0; k < 3; ++k) yf.f[k] = P[k];
     c->g_cy = dim3(nx * yf.ztiles, 1, 1);
 
     XPassArgs& xp = c->a_xp;
-    xp.nx = nx; xp.ny = ny; xp.nz = nz; xp.sx = sx; xp.sy = sy;
-    xp.Pc = pick_pc(kTileMax / 2, nx, nz);
-    xp.ztiles = (nz + xp.Pc - 1) / xp.Pc;
+    xp.ny = ny; xp.nz = nz; xp.sx = sx; xp.sy = sy;
+    xp.ztiles = (nz + Px - 1) / Px;
     xp.shot_stride = N;
-    xp.plan = X.plan; xp.tw = X.tw;
+    xp.tw = X.tw;
     xp.kfx = X.kf; xp.kfy = Y.kf; xp.kfz = Z.kf;
     xp.invN = 1.0 / (double)N;
     xp.ncomp = 3;
@@ -519,43 +511,44 @@ static int build_launch_config(qvti_ctx* c)
 
     CStridedArgs& yi = c->a_yi;
     yi = yf;
-    yi.sign = 1.f;
     yi.nfields = 6;
     for (int k = 0; k < 3; ++k) { yi.f[k] = Q1[k]; yi.f[3 + k] = Q3[k]; }
 
     ZRowsArgs& zi = c->a_zi;
-    zi = zf;
+    zi.nrows = c->nrows;
+    zi.shot_stride = N;
+    zi.tw = Z.tw;
     zi.mode = 1;
-    zi.sign = 1.f;
+    zi.dir = 1;
     zi.nfields = 6;
     for (int k = 0; k < 3; ++k) {
-        zi.cin[k] = Q1[k]; zi.re_out[k] = H1v[k]; zi.im_out[k] = H1w[k];
+        zi.cin[k] = Q1[k]; zi.re_out[k] = H1v[k]; zi.im_out[k] = H1dv[k];
         zi.cin[3 + k] = Q3[k]; zi.re_out[3 + k] = H3v[k]; zi.im_out[3 + k] = nullptr;
     }
 
     // ---- stress x / y derivatives of the 15 family fields
     const float* fam[3][5];
     for (int k = 0; k < 3; ++k) {
-        fam[k][0] = H1v[k]; fam[k][1] = vv[k]; fam[k][2] = H3v[k]; fam[k][3] = H1w[k]; fam[k][4] = ww[k];
+        fam[k][0] = H1v[k]; fam[k][1] = vv[k]; fam[k][2] = H3v[k]; fam[k][3] = H1dv[k]; fam[k][4] = dvv[k];
     }
     float* DX[15];
     float* DY[15];
     for (int k = 0; k < 15; ++k) { DX[k] = fld(c->bufA, k, c); DY[k] = fld(c->bufB, k, c); }
     StridedArgs& sxa = c->a_sx;
-    strided_common(sxa, 0);
+    strided_common(sxa, 0, Px);
     sxa.mode = MODE_STORE;
     sxa.njobs = 15;
     for (int j = 0; j < 5; ++j) {
-        sxa.job[j] = job(fam[0][j], DX[j], nullptr, nullptr, 0, 0);        // ex: dx vx-family, D+
+        sxa.job[j] = job(fam[0][j], DX[j], nullptr, nullptr, 0, 0);          // ex: dx vx-family, D+
         sxa.job[5 + j] = job(fam[2][j], DX[5 + j], nullptr, nullptr, 1, 0);   // gx: dx vz-family, D-
         sxa.job[10 + j] = job(fam[1][j], DX[10 + j], nullptr, nullptr, 1, 0); // hx: dx vy-family, D-
     }
     StridedArgs& sya = c->a_sy;
-    strided_common(sya, 1);
+    strided_common(sya, 1, Py);
     sya.mode = MODE_STORE;
     sya.njobs = 15;
     for (int j = 0; j < 5; ++j) {
-        sya.job[j] = job(fam[1][j], DY[j], nullptr, nullptr, 0, 0);        // ey: dy vy-family, D+
+        sya.job[j] = job(fam[1][j], DY[j], nullptr, nullptr, 0, 0);          // ey: dy vy-family, D+
         sya.job[5 + j] = job(fam[2][j], DY[5 + j], nullptr, nullptr, 1, 0);   // gy: dy vz-family, D-
         sya.job[10 + j] = job(fam[0][j], DY[10 + j], nullptr, nullptr, 1, 0); // hy: dy vx-family, D-
     }
@@ -563,12 +556,11 @@ static int build_launch_config(qvti_ctx* c)
     // ---- stress z + update
     StressZArgs& sz = c->a_sz;
     sz.nx = nx; sz.ny = ny; sz.nz = nz; sz.pml = g.pml;
-    sz.Pc = std::max(1, std::min(std::min(512 / nz, 32), (int)((c->nrows + 1) / 2)));
     sz.nrows = c->nrows;
     sz.dt = g.dt;
     sz.shot_stride = N;
     sz.slabx_stride = c->slab[0]; sz.slaby_stride = c->slab[1]; sz.slabz_stride = c->slab[2];
-    sz.plan = Z.plan; sz.tw = Z.tw; sz.symp = Z.dp; sz.symm = Z.dm;
+    sz.tw = Z.tw; sz.symp = Z.dp; sz.symm = Z.dm;
     sz.pax = X.a; sz.pbx = X.b; sz.paxh = X.ah; sz.pbxh = X.bh;
     sz.pay = Y.a; sz.pby = Y.b; sz.payh = Y.ah; sz.pbyh = Y.bh;
     sz.paz = Z.a; sz.pbz = Z.b; sz.pazh = Z.ah; sz.pbzh = Z.bh;
@@ -594,14 +586,15 @@ static int build_launch_config(qvti_ctx* c)
     sz.src_ip = c->d_src_ip;
     sz.rik = nullptr;
     sz.it = c->d_it;
-    c->g_sz = dim3((unsigned)((c->nrows + 2 * sz.Pc - 1) / (2 * sz.Pc)), 1, 1);
+    const int R = Pz / 8;
+    c->g_sz = dim3((unsigned)((c->nrows + R - 1) / R), 1, 1);
 
     RecArgs& rec = c->a_rec;
     rec.vx = vv[0]; rec.vy = vv[1]; rec.vz = vv[2];
     rec.shot_stride = N;
     rec.rec_off = c->d_rec_off;
     rec.it = c->d_it;
-    (void)pxx; (void)pyy;
+    (void)pyy;
     return QVTI_OK;
 }
 
@@ -669,23 +662,43 @@ static void enqueue_step(qvti_ctx* c)
     hipStream_t st = c->stream;
     const dim3 blk(kThreads);
     const unsigned S = (unsigned)c->nshots;
+    const int nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
     auto timed = [&](int cls, auto&& launch) {
         const int e0 = c->timing ? record_event(c, cls) : -1;
         launch();
         if (e0 >= 0) record_event(c, -1 - cls);
     };
     timed(KC_TICK, [&] { hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, c->d_it); });
-    timed(KC_VEL_X, [&] { hipLaunchKernelGGL(k_deriv_strided, dim3(c->g_vx.x, S), blk, 0, st, c->a_vx); });
-    timed(KC_VEL_Y, [&] { hipLaunchKernelGGL(k_deriv_strided, dim3(c->g_vy.x, S), blk, 0, st, c->a_vy); });
-    timed(KC_VEL_Z, [&] { hipLaunchKernelGGL(k_vel_z, dim3(c->g_vz.x, S), blk, 0, st, c->a_vz); });
-    timed(KC_Z_FWD, [&] { hipLaunchKernelGGL(k_zrows, dim3(c->g_zr.x, S), blk, 0, st, c->a_zf); });
-    timed(KC_Y_FWD, [&] { hipLaunchKernelGGL(k_cfft_strided, dim3(c->g_cy.x, S), blk, 0, st, c->a_yf); });
-    timed(KC_XPASS, [&] { hipLaunchKernelGGL(k_xpass, dim3(c->g_xp.x, S), blk, 0, st, c->a_xp); });
-    timed(KC_Y_INV, [&] { hipLaunchKernelGGL(k_cfft_strided, dim3(c->g_cy.x, S), blk, 0, st, c->a_yi); });
-    timed(KC_Z_INV, [&] { hipLaunchKernelGGL(k_zrows, dim3(c->g_zr.x, S), blk, 0, st, c->a_zi); });
-    timed(KC_STR_X, [&] { hipLaunchKernelGGL(k_deriv_strided, dim3(c->g_vx.x, S), blk, 0, st, c->a_sx); });
-    timed(KC_STR_Y, [&] { hipLaunchKernelGGL(k_deriv_strided, dim3(c->g_vy.x, S), blk, 0, st, c->a_sy); });
-    timed(KC_STR_Z, [&] { hipLaunchKernelGGL(k_stress_z, dim3(c->g_sz.x, S), blk, 0, st, c->a_sz); });
+    timed(KC_VEL_X, [&] {
+        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(c->g_vx.x, S), blk, 0, st, c->a_vx); });
+    });
+    timed(KC_VEL_Y, [&] {
+        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(c->g_vy.x, S), blk, 0, st, c->a_vy); });
+    });
+    timed(KC_VEL_Z, [&] {
+        dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_vel_z<decltype(g)>, dim3(c->g_vz.x, S), blk, 0, st, c->a_vz); });
+    });
+    timed(KC_Y_FWD, [&] {
+        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL((k2_cfft_strided<decltype(g), -1>), dim3(c->g_cy.x, S), blk, 0, st, c->a_yf); });
+    });
+    timed(KC_XPASS, [&] {
+        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL(k2_xpass<decltype(g)>, dim3(c->g_xp.x, S), blk, 0, st, c->a_xp); });
+    });
+    timed(KC_Y_INV, [&] {
+        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL((k2_cfft_strided<decltype(g), 1>), dim3(c->g_cy.x, S), blk, 0, st, c->a_yi); });
+    });
+    timed(KC_Z_INV, [&] {
+        dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3(c->g_zr.x, S), blk, 0, st, c->a_zi); });
+    });
+    timed(KC_STR_X, [&] {
+        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(c->g_vx.x, S), blk, 0, st, c->a_sx); });
+    });
+    timed(KC_STR_Y, [&] {
+        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(c->g_vy.x, S), blk, 0, st, c->a_sy); });
+    });
+    timed(KC_STR_Z, [&] {
+        dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_stress_z<decltype(g)>, dim3(c->g_sz.x, S), blk, 0, st, c->a_sz); });
+    });
     if (c->total_rec > 0)
         timed(KC_RECORD, [&] { hipLaunchKernelGGL(k_record, dim3(c->g_rec.x, S), blk, 0, st, c->a_rec); });
 }
@@ -1059,8 +1072,7 @@ extern "C" int qvti_kernel_bytes(qvti_ctx* c, int max_classes, double* bytes)
     b[KC_TICK] = 8;
     b[KC_VEL_X] = 6 * f + 3 * 2 * sx;
     b[KC_VEL_Y] = 9 * f + 3 * 2 * sy;
-    b[KC_VEL_Z] = (3 + 3 + 3 + 3 + 3) * f + 3.0 * 4 * c->N + 3 * 2 * sz;  // sig, acc, v r/w, w, buoy
-    b[KC_Z_FWD] = 6 * f + 6 * f;
+    b[KC_VEL_Z] = (3 + 3 + 3 + 3 + 3 + 6) * f + 3.0 * 4 * c->N + 3 * 2 * sz;  // sig, acc, v r/w, dv, P, buoy
     b[KC_Y_FWD] = 12 * f;
     b[KC_XPASS] = 6 * f + 12 * f;
     b[KC_Y_INV] = 24 * f;
@@ -1075,24 +1087,30 @@ extern "C" int qvti_kernel_bytes(qvti_ctx* c, int max_classes, double* bytes)
 }
 
 // ------------------------------------------------ standalone transforms ---
+static std::vector<float2> twiddles(int n)
+{
+    std::vector<float2> t(n);
+    for (int i = 0; i < n; ++i) {
+        const double ang = -2.0 * M_PI * i / n;
+        t[i] = make_float2((float)cos(ang), (float)sin(ang));
+    }
+    return t;
+}
+
 extern "C" int qvti_fft3d(int nx, int ny, int nz, int sign, float* host_data)
 {
     if (!host_data || (sign != 1 && sign != -1)) return qvti_fail(QVTI_E_ARG, "bad arguments");
-    FftPlan px, py, pz;
-    if (!make_plan(nx, &px) || !make_plan(ny, &py) || !make_plan(nz, &pz))
-        return qvti_fail(QVTI_E_UNSUPPORTED, "lengths must be 2/3/5-smooth and <= %d", kMaxN);
+    const int dims[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a)
+        if (!supported_n(dims[a]))
+            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d: supported lengths are %s", a, dims[a], kSupportedList);
     const long long N = (long long)nx * ny * nz;
     float2* d = nullptr;
     float2* tw[3] = {nullptr, nullptr, nullptr};
-    const int dims[3] = {nx, ny, nz};
     HIPCHK(hipMalloc(&d, sizeof(float2) * N));
     int rc = QVTI_OK;
     for (int a = 0; a < 3 && !rc; ++a) {
-        std::vector<float2> t(dims[a]);
-        for (int i = 0; i < dims[a]; ++i) {
-            const double ang = -2.0 * M_PI * i / dims[a];
-            t[i] = make_float2((float)cos(ang), (float)sin(ang));
-        }
+        std::vector<float2> t = twiddles(dims[a]);
         if (hipMalloc(&tw[a], sizeof(float2) * dims[a]) != hipSuccess ||
             hipMemcpy(tw[a], t.data(), sizeof(float2) * dims[a], hipMemcpyHostToDevice) != hipSuccess)
             rc = qvti_fail(QVTI_E_DEVICE, "table upload failed");
@@ -1102,17 +1120,22 @@ extern "C" int qvti_fft3d(int nx, int ny, int nz, int sign, float* host_data)
     if (!rc) {
         const long long nrows = (long long)nx * ny;
         ZRowsArgs z{};
-        z.nz = nz; z.Pc = pick_pc(kTileMax, nz, (int)std::min<long long>(nrows, 64)); z.sign = (float)sign;
-        z.nrows = nrows; z.shot_stride = N; z.plan = pz; z.tw = tw[2]; z.mode = 2; z.nfields = 1;
+        z.nrows = nrows; z.shot_stride = N; z.tw = tw[2]; z.mode = 2; z.dir = sign; z.nfields = 1;
         z.cin[0] = d; z.cout[0] = d;
-        hipLaunchKernelGGL(k_zrows, dim3((unsigned)((nrows + z.Pc - 1) / z.Pc), 1), dim3(kThreads), 0, 0, z);
+        const int Pz = tile_pencils(nz);
+        dispatch_n(nz, [&](auto g) {
+            hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3((unsigned)((nrows + Pz - 1) / Pz), 1), dim3(kThreads), 0, 0, z);
+        });
         for (int axis = 1; axis >= 0; --axis) {
             CStridedArgs y{};
-            y.nz = nz; y.sx = (long long)ny * nz; y.sy = nz; y.axis = axis; y.n = dims[axis];
-            y.Pc = pick_pc(kTileMax, y.n, nz); y.ztiles = (nz + y.Pc - 1) / y.Pc; y.sign = (float)sign;
-            y.shot_stride = N; y.plan = axis ? py : px; y.tw = tw[axis]; y.nfields = 1; y.f[0] = d;
+            const int P = tile_pencils(dims[axis]);
+            y.nz = nz; y.sx = (long long)ny * nz; y.sy = nz; y.axis = axis;
+            y.ztiles = (nz + P - 1) / P; y.shot_stride = N; y.tw = tw[axis]; y.nfields = 1; y.f[0] = d;
             const int other = axis ? nx : ny;
-            hipLaunchKernelGGL(k_cfft_strided, dim3(other * y.ztiles, 1), dim3(kThreads), 0, 0, y);
+            dispatch_n(dims[axis], [&](auto g) {
+                if (sign < 0) hipLaunchKernelGGL((k2_cfft_strided<decltype(g), -1>), dim3(other * y.ztiles, 1), dim3(kThreads), 0, 0, y);
+                else hipLaunchKernelGGL((k2_cfft_strided<decltype(g), 1>), dim3(other * y.ztiles, 1), dim3(kThreads), 0, 0, y);
+            });
         }
         if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
             rc = qvti_fail(QVTI_E_DEVICE, "fft kernels failed");
@@ -1130,16 +1153,11 @@ extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym,
         return qvti_fail(QVTI_E_ARG, "bad arguments");
     const int dims[3] = {nx, ny, nz};
     for (int a = 0; a < 3; ++a)
-        if (!supported_n(dims[a])) return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d unsupported", a, dims[a]);
+        if (!supported_n(dims[a]))
+            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d: supported lengths are %s", a, dims[a], kSupportedList);
     const int n = dims[axis];
-    FftPlan pl;
-    make_plan(n, &pl);
-    std::vector<float2> tw(n), dp, dm;
+    std::vector<float2> tw = twiddles(n), dp, dm;
     std::vector<float> kf;
-    for (int i = 0; i < n; ++i) {
-        const double ang = -2.0 * M_PI * i / n;
-        tw[i] = make_float2((float)cos(ang), (float)sin(ang));
-    }
     axis_symbols(n, dd, dp, dm, kf);
     const long long N = (long long)nx * ny * nz;
     float *din = nullptr, *dout = nullptr;
@@ -1153,24 +1171,28 @@ extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym,
                 hipMemcpy(dsym, (sym ? dm : dp).data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess))
         rc = qvti_fail(QVTI_E_DEVICE, "H2D failed");
     if (!rc) {
+        const int P = tile_pencils(n);
         if (axis < 2) {
             StridedArgs a{};
             a.nx = nx; a.ny = ny; a.nz = nz; a.pml = 0;
-            a.sx = (long long)ny * nz; a.sy = nz; a.axis = axis; a.n = n;
-            a.Pc = pick_pc(kTileMax, n, nz / 2); a.ztiles = (nz + 2 * a.Pc - 1) / (2 * a.Pc);
+            a.sx = (long long)ny * nz; a.sy = nz; a.axis = axis;
+            a.ztiles = (nz + 2 * P - 1) / (2 * P);
             a.mode = MODE_STORE; a.dt = 0.f; a.shot_stride = N; a.slab_stride = 0;
-            a.plan = pl; a.tw = dtw; a.sym[0] = dsym; a.sym[1] = dsym;
+            a.tw = dtw; a.sym[0] = dsym; a.sym[1] = dsym;
             a.njobs = 1;
             a.job[0].in = din; a.job[0].out = dout; a.job[0].acc = nullptr; a.job[0].phi = nullptr;
             a.job[0].sym = 0; a.job[0].half = 0;
             const int other = axis == 0 ? ny : nx;
-            hipLaunchKernelGGL(k_deriv_strided, dim3(other * a.ztiles, 1), dim3(kThreads), 0, 0, a);
+            dispatch_n(n, [&](auto g) {
+                hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(other * a.ztiles, 1), dim3(kThreads), 0, 0, a);
+            });
         } else {
             RowDerivArgs r{};
             const long long nrows = (long long)nx * ny;
-            r.nz = nz; r.Pc = pick_pc(kTileMax, nz, (int)std::min<long long>((nrows + 1) / 2, 64));
-            r.nrows = nrows; r.plan = pl; r.tw = dtw; r.sym = dsym; r.in = din; r.out = dout;
-            hipLaunchKernelGGL(k_deriv_rows, dim3((unsigned)((nrows + 2 * r.Pc - 1) / (2 * r.Pc))), dim3(kThreads), 0, 0, r);
+            r.nrows = nrows; r.tw = dtw; r.sym = dsym; r.in = din; r.out = dout;
+            dispatch_n(n, [&](auto g) {
+                hipLaunchKernelGGL(k2_deriv_rows<decltype(g)>, dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(kThreads), 0, 0, r);
+            });
         }
         if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
             rc = qvti_fail(QVTI_E_DEVICE, "derivative kernel failed");

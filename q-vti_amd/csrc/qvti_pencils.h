@@ -1,0 +1,777 @@
+// qvti_pencils.h -- pencil kernels of the time step on the two-level register
+// FFT (qvti_fft2l.h).  Every kernel is a template over the split G = L2<N1,N2>
+// of its axis length and runs 256 threads.
+//
+// Per step (reference loop: cuviscoelas_3D.cu:1255-1605):
+//   velocity half-step   k2_deriv_strided(x, VEL_FIRST) -> k2_deriv_strided(y, VEL_ACC)
+//                        -> k2_vel_z (z-derivatives, velocity update, forward z-FFT of v + i dv)
+//   stress half-step     k2_cfft_strided(y, fwd) -> k2_xpass -> k2_cfft_strided(y, inv)
+//                        -> k2_zrows_inv -> k2_deriv_strided(x, STORE) -> k2_deriv_strided(y, STORE)
+//                        -> k2_stress_z (z-derivatives, stress update, source injection)
+//   receivers            k_record
+//
+// Algebra used (exact in exact arithmetic; only rounding differs from the
+// reference's C2C + `.x / ntp` formulation):
+//  * every spectral operator of the scheme is Hermitian (real |k|^p times a
+//    1-D shift-derivative whose Nyquist symbol is taken as its real part,
+//    which is what `.x` of the C2C result keeps), so two real fields packed
+//    as a + ib go through ONE complex transform and come back as op(a) + i op(b);
+//  * an operator acting along one axis only (all nine velocity-step
+//    derivatives and the A2 = 1 stress terms) is a 1-D transform along that
+//    axis: the other two axes' forward and inverse transforms cancel;
+//  * A_j D_a V = D_a (A_j V): the 45 stress terms are 1-D derivatives of the
+//    15 fields {A1 v, v, A3 v, A1 dv, dv} per component, with dv = v - v_prev
+//    in real space (v_prev: the previous step's pre-injection velocity) and
+//    the 1/dt of the time difference folded into the tau coefficients, so
+//    only A1 (v + i dv) and A3 v need the full 3-D transform.  v and dv are
+//    packed together because their magnitudes are comparable (|dv| ~ w dt |v|);
+//    packing v with dv/dt instead lets the FP32 rounding of the larger part
+//    swamp the smaller one.
+#pragma once
+#include "qvti_fft2l.h"
+
+namespace qvti {
+
+constexpr int kThreads = 256;
+enum { MODE_STORE = 0, MODE_VEL_FIRST = 1, MODE_VEL_ACC = 2 };
+
+template <class G>
+__device__ __forceinline__ void load_twiddles(float2* tws, const float2* tw)
+{
+    for (int t = threadIdx.x; t < G::N; t += kThreads) tws[t] = tw[t];
+}
+
+// ------------------------------------------------------------------------
+// 1-D shift-derivatives of real fields along a strided axis (x or y).  Tile:
+// G::P complex pencils = 2*G::P adjacent real z-columns at one value of the
+// other in-plane index, loaded as float2 (two real pencils per complex one).
+// Modes: STORE out = d; VEL_FIRST out = d + phi; VEL_ACC out = acc + d + phi,
+// with the velocity C-PML memory variable phi updated on the axis' face slabs
+// (cuviscoelas_3D.cu:1049-1074).
+struct DJob {
+    const float* in;
+    float* out;
+    const float* acc;
+    float* phi;
+    int sym;   // 0: D+, 1: D-
+    int half;  // 1: half-cell averaged C-PML a, b
+};
+
+struct StridedArgs {
+    int nx, ny, nz, pml;
+    long long sx, sy;
+    int axis;  // 0: x (stride sx), 1: y (stride sy)
+    int ztiles;
+    int mode;
+    float dt;
+    long long shot_stride, slab_stride;
+    const float2* tw;
+    const float2* sym[2];
+    const float *pa, *pb, *pah, *pbh;
+    int njobs;
+    DJob job[15];
+};
+
+template <class G>
+__global__ __launch_bounds__(kThreads) void k2_deriv_strided(StridedArgs a)
+{
+    __shared__ float2 lds[G::P * G::PS];
+    __shared__ float2 tws[G::N];
+    const int tid = threadIdx.x, shot = blockIdx.y;
+    const int other = blockIdx.x / a.ztiles;
+    const int p = tid % G::P, jj = tid / G::P;
+    const int zc = (blockIdx.x - other * a.ztiles) * 2 * G::P + 2 * p;
+    const int nz = a.nz;
+    const bool zok = zc < nz;
+    const bool act1 = jj < G::N2, act2 = jj < G::N1;
+    const long long astride = a.axis == 0 ? a.sx : a.sy;
+    const long long obase = (a.axis == 0 ? (long long)other * a.sy : (long long)other * a.sx) + zc;
+    const long long soff = (long long)shot * a.shot_stride;
+    float2* pen = lds + p * G::PS;
+    load_twiddles<G>(tws, a.tw);
+    for (int jb = 0; jb < a.njobs; ++jb) {
+        const DJob J = a.job[jb];
+        __syncthreads();
+        if (act1) {
+            float2 v[G::N1];
+            const float* in = J.in + soff + obase;
+#pragma unroll
+            for (int j1 = 0; j1 < G::N1; ++j1)
+                v[j1] = zok ? *reinterpret_cast<const float2*>(in + (long long)(j1 * G::N2 + jj) * astride)
+                            : make_float2(0.f, 0.f);
+            fwd_l1<G>(v, pen, tws, jj);
+        }
+        __syncthreads();
+        float2 b[G::N2];
+        if (act2) {
+            fwd_l2<G>(b, pen, jj);
+            const float2* S = a.sym[J.sym];
+#pragma unroll
+            for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = c_mul(b[k2], S[jj + G::N1 * k2]);
+        }
+        __syncthreads();
+        if (act2) inv_la<G>(b, pen, tws, jj);
+        __syncthreads();
+        if (act1 && zok) {
+            float2 d[G::N1];
+            inv_lb<G>(d, pen, jj);
+#pragma unroll
+            for (int j1 = 0; j1 < G::N1; ++j1) {
+                const int i = jj + G::N2 * j1;
+                const long long g = soff + obase + (long long)i * astride;
+                float2 r = d[j1];
+                if (a.mode != MODE_STORE) {
+                    const int ix = a.axis == 0 ? i : other;
+                    const int iy = a.axis == 0 ? other : i;
+                    const int pml = a.pml, n = G::N;
+                    if ((i < pml || i >= n - pml) && ix < a.nx - 1 && iy < a.ny - 1) {
+                        const int si = i < pml ? i : i - (n - 2 * pml);
+                        const long long sidx = a.axis == 0 ? ((long long)si * a.ny + iy) * nz + zc
+                                                           : ((long long)ix * 2 * pml + si) * nz + zc;
+                        float2* ph = reinterpret_cast<float2*>(J.phi + (long long)shot * a.slab_stride + sidx);
+                        const float A = J.half ? a.pah[i] : a.pa[i];
+                        const float B = J.half ? a.pbh[i] : a.pb[i];
+                        float2 q = *ph;
+                        if (zc < nz - 1) { q.x = q.x + a.dt * (-A * q.x - B * r.x); r.x = r.x + q.x; }
+                        if (zc + 1 < nz - 1) { q.y = q.y + a.dt * (-A * q.y - B * r.y); r.y = r.y + q.y; }
+                        *ph = q;
+                    }
+                    if (a.mode == MODE_VEL_ACC) {
+                        const float2 ac = *reinterpret_cast<const float2*>(J.acc + g);
+                        r = make_float2(ac.x + r.x, ac.y + r.y);
+                    }
+                }
+                *reinterpret_cast<float2*>(J.out + g) = r;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Velocity update (cuviscoelas_3D.cu:1018-1101) fused with the z-derivatives
+// and with the first (z) pass of the stress step's 3-D transform:
+//   v += b * (acc_xy + dz sigma + phi_z);  dv = v_new - v_old + src_prev;
+//   P_c = FFT_z(v_c + i dv_c)
+// dv is the change since the previous step's pre-injection velocity (the
+// reference's V - V_prev, cuviscoelas_3D.cu:517-522, 1124-1131, without 1/dt).
+// Tile: G::P rows (z-pencils).
+struct VelZArgs {
+    int nx, ny, nz, pml;
+    long long nrows;
+    float dt;
+    long long shot_stride, slab_stride;
+    const float2* tw;
+    const float2 *symp, *symm;
+    const float *pa, *pb, *pah, *pbh;
+    const float *pxz, *pyz, *pzz;
+    const float* acc[3];
+    float* v[3];
+    float* dv[3];
+    float2* P[3];
+    const float* buoy[3];
+    float* phi[3];  // z-slab: phi_pxz_z (node), phi_pyz_z (node), phi_pzz_z (half)
+    const int* src_kind;
+    const long long* src_ip;
+    const float* rik;
+    const int* it;
+};
+
+template <class G>
+__global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
+{
+    __shared__ float2 lds[G::P * G::PS];
+    __shared__ float2 tws[G::N];
+    constexpr int n = G::N;
+    const int tid = threadIdx.x, shot = blockIdx.y;
+    // level 1/B mapping (column j fastest) and level 2/A mapping (row k1 fastest)
+    const int j = tid % G::N2, p1 = tid / G::N2;
+    const int k1 = tid % G::N1, p2 = tid / G::N1;
+    const bool act1 = p1 < G::P, act2 = p2 < G::P;
+    const long long r1 = (long long)blockIdx.x * G::P + p1, r2 = (long long)blockIdx.x * G::P + p2;
+    const bool ok1 = act1 && r1 < a.nrows, ok2 = act2 && r2 < a.nrows;
+    const long long soff = (long long)shot * a.shot_stride;
+    float2* pen1 = lds + p1 * G::PS;
+    float2* pen2 = lds + p2 * G::PS;
+    load_twiddles<G>(tws, a.tw);
+
+    // derivative of (fa + i fb) along z with symbol S, result in d (level-1 layout)
+    auto deriv = [&](const float* fa, const float* fb, const float2* S, float2 (&d)[G::N1]) {
+        __syncthreads();
+        if (act1) {
+            float2 v[G::N1];
+#pragma unroll
+            for (int j1 = 0; j1 < G::N1; ++j1) {
+                const long long g = soff + r1 * n + j1 * G::N2 + j;
+                v[j1] = ok1 ? make_float2(fa[g], fb ? fb[g] : 0.f) : make_float2(0.f, 0.f);
+            }
+            fwd_l1<G>(v, pen1, tws, j);
+        }
+        __syncthreads();
+        float2 b[G::N2];
+        if (act2) {
+            fwd_l2<G>(b, pen2, k1);
+#pragma unroll
+            for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = c_mul(b[k2], S[k1 + G::N1 * k2]);
+        }
+        __syncthreads();
+        if (act2) inv_la<G>(b, pen2, tws, k1);
+        __syncthreads();
+        if (act1) inv_lb<G>(d, pen1, j);
+    };
+    float2 dA[G::N1], dB[G::N1];
+    deriv(a.pxz, a.pyz, a.symp, dA);
+    deriv(a.pzz, nullptr, a.symm, dB);
+
+    const int it = *a.it;
+    const int kind = a.src_kind[shot];
+    const long long sip = a.src_ip[shot];
+    const float sprev = it > 0 ? a.rik[it - 1] : 0.f;
+    const int ix = (int)(r1 / a.ny), iy = (int)(r1 - (long long)ix * a.ny);
+    const bool rowdom = ok1 && ix < a.nx - 1 && iy < a.ny - 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float2 x[G::N1];
+#pragma unroll
+        for (int j1 = 0; j1 < G::N1; ++j1) {
+            const int iz = j + G::N2 * j1;
+            const long long ip = r1 * n + iz;
+            float d = c == 0 ? dA[j1].x : (c == 1 ? dA[j1].y : dB[j1].x);
+            float vold = 0.f, vnew = 0.f;
+            if (ok1) {
+                vold = a.v[c][soff + ip];
+                vnew = vold;
+                if (rowdom && iz < n - 1) {
+                    if (iz < a.pml || iz >= n - a.pml) {
+                        const int si = iz < a.pml ? iz : iz - (n - 2 * a.pml);
+                        float* ph = a.phi[c] + (long long)shot * a.slab_stride + r1 * 2 * a.pml + si;
+                        const bool h = c == 2;
+                        const float A = h ? a.pah[iz] : a.pa[iz];
+                        const float B = h ? a.pbh[iz] : a.pb[iz];
+                        const float q = *ph + a.dt * (-A * *ph - B * d);
+                        *ph = q;
+                        d = d + q;
+                    }
+                    vnew = vold + a.buoy[c][ip] * (a.acc[c][soff + ip] + d);
+                    a.v[c][soff + ip] = vnew;
+                }
+                float dv = vnew - vold;
+                if (ip == sip && kind == c) dv += sprev;
+                a.dv[c][soff + ip] = dv;
+                x[j1] = make_float2(vnew, dv);
+            } else {
+                x[j1] = make_float2(0.f, 0.f);
+            }
+        }
+        // forward z-FFT of v + i dv (first pass of the 3-D transform)
+        __syncthreads();
+        if (act1) fwd_l1<G>(x, pen1, tws, j);
+        __syncthreads();
+        if (ok2) {
+            float2 b[G::N2];
+            fwd_l2<G>(b, pen2, k1);
+            float2* dst = a.P[c] + soff + r2 * n;
+#pragma unroll
+            for (int k2 = 0; k2 < G::N2; ++k2) dst[k1 + G::N1 * k2] = b[k2];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// In-place complex transforms along a strided axis (y: stride sy, x: stride
+// sx) for up to 6 complex fields.  Tile: G::P adjacent complex z-columns.
+// DIR -1: forward (natural-order input, natural-order spectrum out);
+// DIR +1: inverse.
+struct CStridedArgs {
+    int nz;
+    long long sx, sy;
+    int axis, ztiles;
+    long long shot_stride;
+    const float2* tw;
+    int nfields;
+    float2* f[6];
+};
+
+template <class G, int DIR>
+__global__ __launch_bounds__(kThreads) void k2_cfft_strided(CStridedArgs a)
+{
+    __shared__ float2 lds[G::P * G::PS];
+    __shared__ float2 tws[G::N];
+    const int tid = threadIdx.x, shot = blockIdx.y;
+    const int other = blockIdx.x / a.ztiles;
+    const int p = tid % G::P, jj = tid / G::P;
+    const int zc = (blockIdx.x - other * a.ztiles) * G::P + p;
+    const bool zok = zc < a.nz;
+    const bool act1 = jj < G::N2 && zok, act2 = jj < G::N1 && zok;
+    const long long astride = a.axis == 0 ? a.sx : a.sy;
+    const long long obase = (a.axis == 0 ? (long long)other * a.sy : (long long)other * a.sx) + zc;
+    const long long soff = (long long)shot * a.shot_stride;
+    float2* pen = lds + p * G::PS;
+    load_twiddles<G>(tws, a.tw);
+    for (int f = 0; f < a.nfields; ++f) {
+        float2* base = a.f[f] + soff + obase;
+        __syncthreads();
+        if (DIR < 0) {
+            if (act1) {
+                float2 v[G::N1];
+#pragma unroll
+                for (int j1 = 0; j1 < G::N1; ++j1) v[j1] = base[(long long)(j1 * G::N2 + jj) * astride];
+                fwd_l1<G>(v, pen, tws, jj);
+            }
+            __syncthreads();
+            if (act2) {
+                float2 b[G::N2];
+                fwd_l2<G>(b, pen, jj);
+#pragma unroll
+                for (int k2 = 0; k2 < G::N2; ++k2) base[(long long)(jj + G::N1 * k2) * astride] = b[k2];
+            }
+        } else {
+            if (act2) {
+                float2 b[G::N2];
+#pragma unroll
+                for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = base[(long long)(jj + G::N1 * k2) * astride];
+                inv_la<G>(b, pen, tws, jj);
+            }
+            __syncthreads();
+            if (act1) {
+                float2 d[G::N1];
+                inv_lb<G>(d, pen, jj);
+#pragma unroll
+                for (int j1 = 0; j1 < G::N1; ++j1) base[(long long)(jj + G::N2 * j1) * astride] = d[j1];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// x-pass of the 3-D transform: forward x-FFT of P (already z- and
+// y-transformed), multiply by |k|^-1 / N and |k| / N (|k|^2 = kx^2+ky^2+kz^2
+// from the folded 1-D |K| tables, evaluated in FP64, cuviscoelas_3D.cu:72,
+// 188-190; the DC symbol of |k|^-1 is 0 because every use is followed by a
+// derivative that vanishes there), inverse x-FFT, write Q1, Q3.
+struct XPassArgs {
+    int ny, nz;
+    long long sx, sy;
+    int ztiles;
+    long long shot_stride;
+    const float2* tw;
+    const float *kfx, *kfy, *kfz;
+    double invN;
+    int ncomp;
+    const float2* P[3];
+    float2* Q1[3];
+    float2* Q3[3];
+};
+
+template <class G>
+__global__ __launch_bounds__(kThreads) void k2_xpass(XPassArgs a)
+{
+    __shared__ float2 lds[G::P * G::PS];
+    __shared__ float2 tws[G::N];
+    const int tid = threadIdx.x, shot = blockIdx.y;
+    const int iy = blockIdx.x / a.ztiles;
+    const int p = tid % G::P, jj = tid / G::P;
+    const int zc = (blockIdx.x - iy * a.ztiles) * G::P + p;
+    const bool zok = zc < a.nz;
+    const bool act1 = jj < G::N2 && zok, act2 = jj < G::N1 && zok;
+    const long long obase = (long long)iy * a.sy + zc;
+    const long long soff = (long long)shot * a.shot_stride;
+    float2* pen = lds + p * G::PS;
+    load_twiddles<G>(tws, a.tw);
+    const double ky = a.kfy[iy];
+    const double kz = a.kfz[zok ? zc : 0];
+    const double kyz = ky * ky + kz * kz;
+    for (int comp = 0; comp < a.ncomp; ++comp) {
+        const float2* src = a.P[comp] + soff + obase;
+        __syncthreads();
+        if (act1) {
+            float2 v[G::N1];
+#pragma unroll
+            for (int j1 = 0; j1 < G::N1; ++j1) v[j1] = src[(long long)(j1 * G::N2 + jj) * a.sx];
+            fwd_l1<G>(v, pen, tws, jj);
+        }
+        __syncthreads();
+        float2 b[G::N2];
+        if (act2) fwd_l2<G>(b, pen, jj);
+#pragma unroll
+        for (int op = 0; op < 2; ++op) {
+            __syncthreads();
+            if (act2) {
+                float2 c[G::N2];
+#pragma unroll
+                for (int k2 = 0; k2 < G::N2; ++k2) {
+                    const double kx = a.kfx[jj + G::N1 * k2];
+                    const double k2s = kx * kx + kyz;
+                    double s;
+                    if (op == 0) s = k2s > 0.0 ? a.invN / sqrt(k2s) : 0.0;
+                    else s = a.invN * sqrt(k2s);
+                    const float fs = (float)s;
+                    c[k2] = make_float2(b[k2].x * fs, b[k2].y * fs);
+                }
+                inv_la<G>(c, pen, tws, jj);
+            }
+            __syncthreads();
+            if (act1) {
+                float2 d[G::N1];
+                inv_lb<G>(d, pen, jj);
+                float2* dst = (op == 0 ? a.Q1[comp] : a.Q3[comp]) + soff + obase;
+#pragma unroll
+                for (int j1 = 0; j1 < G::N1; ++j1) dst[(long long)(jj + G::N2 * j1) * a.sx] = d[j1];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// z-direction complex transforms on rows.
+//  mode 1 (inverse, stress step end): re_out + i im_out = IFFT_z(cin)   (im_out may be null)
+//  mode 2 (plain, standalone transforms): cout = FFT_z^DIR(cin), in place allowed
+struct ZRowsArgs {
+    long long nrows;
+    long long shot_stride;
+    const float2* tw;
+    int mode, dir, nfields;
+    const float2* cin[6];
+    float2* cout[6];
+    float* re_out[6];
+    float* im_out[6];
+};
+
+template <class G>
+__global__ __launch_bounds__(kThreads) void k2_zrows(ZRowsArgs a)
+{
+    __shared__ float2 lds[G::P * G::PS];
+    __shared__ float2 tws[G::N];
+    constexpr int n = G::N;
+    const int tid = threadIdx.x, shot = blockIdx.y;
+    const int j = tid % G::N2, p1 = tid / G::N2;
+    const int k1 = tid % G::N1, p2 = tid / G::N1;
+    const long long r1 = (long long)blockIdx.x * G::P + p1, r2 = (long long)blockIdx.x * G::P + p2;
+    const bool ok1 = p1 < G::P && r1 < a.nrows, ok2 = p2 < G::P && r2 < a.nrows;
+    const long long soff = (long long)shot * a.shot_stride;
+    float2* pen1 = lds + p1 * G::PS;
+    float2* pen2 = lds + p2 * G::PS;
+    load_twiddles<G>(tws, a.tw);
+    for (int f = 0; f < a.nfields; ++f) {
+        __syncthreads();
+        if (a.mode == 2 && a.dir < 0) {
+            if (ok1) {
+                float2 v[G::N1];
+                const float2* src = a.cin[f] + soff + r1 * n;
+#pragma unroll
+                for (int j1 = 0; j1 < G::N1; ++j1) v[j1] = src[j1 * G::N2 + j];
+                fwd_l1<G>(v, pen1, tws, j);
+            }
+            __syncthreads();
+            if (ok2) {
+                float2 b[G::N2];
+                fwd_l2<G>(b, pen2, k1);
+                float2* dst = a.cout[f] + soff + r2 * n;
+#pragma unroll
+                for (int k2 = 0; k2 < G::N2; ++k2) dst[k1 + G::N1 * k2] = b[k2];
+            }
+        } else {
+            if (ok2) {
+                float2 b[G::N2];
+                const float2* src = a.cin[f] + soff + r2 * n;
+#pragma unroll
+                for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = src[k1 + G::N1 * k2];
+                inv_la<G>(b, pen2, tws, k1);
+            }
+            __syncthreads();
+            if (ok1) {
+                float2 d[G::N1];
+                inv_lb<G>(d, pen1, j);
+                if (a.mode == 2) {
+                    float2* dst = a.cout[f] + soff + r1 * n;
+#pragma unroll
+                    for (int j1 = 0; j1 < G::N1; ++j1) dst[j + G::N2 * j1] = d[j1];
+                } else {
+                    float* re = a.re_out[f] + soff + r1 * n;
+                    float* im = a.im_out[f] ? a.im_out[f] + soff + r1 * n : nullptr;
+#pragma unroll
+                    for (int j1 = 0; j1 < G::N1; ++j1) {
+                        re[j + G::N2 * j1] = d[j1].x;
+                        if (im) im[j + G::N2 * j1] = d[j1].y;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// 1-D shift-derivative along z of one real field (standalone entry point).
+// Tile: G::P complex pencils = 2*G::P rows.
+struct RowDerivArgs {
+    long long nrows;
+    const float2* tw;
+    const float2* sym;
+    const float* in;
+    float* out;
+};
+
+template <class G>
+__global__ __launch_bounds__(kThreads) void k2_deriv_rows(RowDerivArgs a)
+{
+    __shared__ float2 lds[G::P * G::PS];
+    __shared__ float2 tws[G::N];
+    constexpr int n = G::N;
+    const int tid = threadIdx.x;
+    const int j = tid % G::N2, p1 = tid / G::N2;
+    const int k1 = tid % G::N1, p2 = tid / G::N1;
+    const long long ra = (long long)blockIdx.x * 2 * G::P + 2 * p1;  // rows ra, ra+1
+    const bool act1 = p1 < G::P, act2 = p2 < G::P;
+    float2* pen1 = lds + p1 * G::PS;
+    float2* pen2 = lds + p2 * G::PS;
+    load_twiddles<G>(tws, a.tw);
+    __syncthreads();
+    if (act1) {
+        float2 v[G::N1];
+#pragma unroll
+        for (int j1 = 0; j1 < G::N1; ++j1) {
+            const int i = j1 * G::N2 + j;
+            v[j1] = make_float2(ra < a.nrows ? a.in[ra * n + i] : 0.f, ra + 1 < a.nrows ? a.in[(ra + 1) * n + i] : 0.f);
+        }
+        fwd_l1<G>(v, pen1, tws, j);
+    }
+    __syncthreads();
+    float2 b[G::N2];
+    if (act2) {
+        fwd_l2<G>(b, pen2, k1);
+#pragma unroll
+        for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = c_mul(b[k2], a.sym[k1 + G::N1 * k2]);
+    }
+    __syncthreads();
+    if (act2) inv_la<G>(b, pen2, tws, k1);
+    __syncthreads();
+    if (act1) {
+        float2 d[G::N1];
+        inv_lb<G>(d, pen1, j);
+#pragma unroll
+        for (int j1 = 0; j1 < G::N1; ++j1) {
+            const int i = j + G::N2 * j1;
+            if (ra < a.nrows) a.out[ra * n + i] = d[j1].x;
+            if (ra + 1 < a.nrows) a.out[(ra + 1) * n + i] = d[j1].y;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Stress update (cuviscoelas_3D.cu:800-1015, generalised to VTI moduli) fused
+// with the z-derivatives of the 15 family fields, the stress C-PML memory
+// variables on the face slabs, and the source injection
+// (cuviscoelas_3D.cu:771-777).  Tile: R = G::P/8 rows; 8 complex pencils per
+// row carry the 15 fields (two per pencil, same z-symbol); their derivatives
+// are staged in LDS, then every thread updates R*nz/256 cells.
+struct StressZArgs {
+    int nx, ny, nz, pml;
+    long long nrows;
+    float dt;
+    long long shot_stride, slabx_stride, slaby_stride, slabz_stride;
+    const float2* tw;
+    const float2 *symp, *symm;
+    const float *pax, *pbx, *paxh, *pbxh;
+    const float *pay, *pby, *payh, *pbyh;
+    const float *paz, *pbz, *pazh, *pbzh;
+    const float* fam[3][5];  // [component][A1 v, v, A3 v, A1 dv, dv]
+    const float *ex[5], *gx[5], *hx[5];
+    const float *ey[5], *gy[5], *hy[5];
+    const float *c11[5], *c33[5], *c13[5], *cs[5], *csb[5];
+    float* sig[6];  // pxx pyy pzz pxy pxz pyz
+    float* phx[5];
+    float* phy[5];
+    float* phz[4];
+    const int* src_kind;
+    const long long* src_ip;
+    const float* rik;
+    const int* it;
+};
+
+// pencil kind -> (component, family member of the real part, of the imaginary part)
+// staging field index = 2*kind (+1 for the imaginary part)
+//   0: A1 vx + i A1 dvx (D-)   1: A1 vy + i A1 dvy (D-)   2: A1 vz + i A1 dvz (D+)
+//   3: vx + i dvx (D-)         4: vy + i dvy (D-)         5: vz + i dvz (D+)
+//   6: A3 vx + i A3 vy (D-)    7: A3 vz (D+)
+template <class G>
+__global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
+{
+    constexpr int R = G::P / 8;
+    constexpr int n = G::N;
+    static_assert(R >= 1, "tile too small");
+    static_assert(G::PS >= n, "staging must fit in the pencil buffer");
+    __shared__ float2 lds[8 * R * G::PS];
+    __shared__ float2 tws[G::N];
+    const int tid = threadIdx.x, shot = blockIdx.y;
+    const int j = tid % G::N2, q1 = tid / G::N2;
+    const int k1 = tid % G::N1, q2 = tid / G::N1;
+    const bool act1 = q1 < 8 * R, act2 = q2 < 8 * R;
+    const long long r0 = (long long)blockIdx.x * R;
+    const long long row1 = r0 + q1 / 8;
+    const bool ok1 = act1 && row1 < a.nrows;
+    const long long soff = (long long)shot * a.shot_stride;
+    float2* pen1 = lds + q1 * G::PS;
+    float2* pen2 = lds + q2 * G::PS;
+    load_twiddles<G>(tws, a.tw);
+    __syncthreads();
+    float2 d[G::N1];
+    {
+        const int kind = q1 % 8;
+        if (act1) {
+            const float* fre;
+            const float* fim;
+            switch (kind) {
+            case 0: fre = a.fam[0][0]; fim = a.fam[0][3]; break;
+            case 1: fre = a.fam[1][0]; fim = a.fam[1][3]; break;
+            case 2: fre = a.fam[2][0]; fim = a.fam[2][3]; break;
+            case 3: fre = a.fam[0][1]; fim = a.fam[0][4]; break;
+            case 4: fre = a.fam[1][1]; fim = a.fam[1][4]; break;
+            case 5: fre = a.fam[2][1]; fim = a.fam[2][4]; break;
+            case 6: fre = a.fam[0][2]; fim = a.fam[1][2]; break;
+            default: fre = a.fam[2][2]; fim = nullptr; break;
+            }
+            float2 v[G::N1];
+#pragma unroll
+            for (int j1 = 0; j1 < G::N1; ++j1) {
+                const long long g = soff + row1 * n + j1 * G::N2 + j;
+                v[j1] = ok1 ? make_float2(fre[g], fim ? fim[g] : 0.f) : make_float2(0.f, 0.f);
+            }
+            fwd_l1<G>(v, pen1, tws, j);
+        }
+        __syncthreads();
+        float2 b[G::N2];
+        if (act2) {
+            fwd_l2<G>(b, pen2, k1);
+            const int k2kind = q2 % 8;
+            const float2* S = (k2kind == 2 || k2kind == 5 || k2kind == 7) ? a.symp : a.symm;
+#pragma unroll
+            for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = c_mul(b[k2], S[k1 + G::N1 * k2]);
+        }
+        __syncthreads();
+        if (act2) inv_la<G>(b, pen2, tws, k1);
+        __syncthreads();
+        if (act1) inv_lb<G>(d, pen1, j);
+        __syncthreads();
+        // stage: st[field][rr][iz], field = 2*kind + part
+        float* st = reinterpret_cast<float*>(lds);
+        if (act1) {
+            const int rr = q1 / 8;
+#pragma unroll
+            for (int j1 = 0; j1 < G::N1; ++j1) {
+                const int iz = j + G::N2 * j1;
+                st[((2 * kind) * R + rr) * n + iz] = d[j1].x;
+                st[((2 * kind + 1) * R + rr) * n + iz] = d[j1].y;
+            }
+        }
+        __syncthreads();
+    }
+    const float* st = reinterpret_cast<const float*>(lds);
+    const int pml = a.pml;
+    const float dt = a.dt;
+    const int it = *a.it;
+    const int kind = a.src_kind[shot];
+    const long long sip = a.src_ip[shot];
+    for (int cell = tid; cell < R * n; cell += kThreads) {
+        const int rr = cell / n, iz = cell - rr * n;
+        const long long row = r0 + rr;
+        if (row >= a.nrows) continue;
+        const int ix = (int)(row / a.ny), iy = (int)(row - (long long)ix * a.ny);
+        const long long ip = row * n + iz;
+        const long long g = soff + ip;
+        auto S = [&](int f) { return st[(f * R + rr) * n + iz]; };
+        if (ix < a.nx - 1 && iy < a.ny - 1 && iz < n - 1) {
+            // staging fields of the family members j = 0..4 (A1 v, v, A3 v, A1 dv, dv)
+            const int fz[5] = {4, 10, 14, 5, 11};   // vz family (D+): normal dz vz
+            const int fgx[5] = {0, 6, 12, 1, 7};    // vx family (D-): dz vx for pxz
+            const int fhy[5] = {2, 8, 13, 3, 9};    // vy family (D-): dz vy for pyz
+            float A[14];
+#pragma unroll
+            for (int k = 0; k < 14; ++k) A[k] = 0.f;
+#pragma unroll
+            for (int jf = 0; jf < 5; ++jf) {
+                const float ez = S(fz[jf]), gz = S(fgx[jf]), hz = S(fhy[jf]);
+                const float ex = a.ex[jf][g], gx = a.gx[jf][g], hx = a.hx[jf][g];
+                const float ey = a.ey[jf][g], gy = a.gy[jf][g], hy = a.hy[jf][g];
+                const float c11 = a.c11[jf][ip], c33 = a.c33[jf][ip], c13 = a.c13[jf][ip];
+                const float cs = a.cs[jf][ip], csb = a.csb[jf][ip];
+                const float c12 = c11 - 2 * cs, c13d = c13 - 2 * cs;
+                A[0] += c11 * ex;   // s11x
+                A[1] += c12 * ex;   // s12x
+                A[2] += c13d * ex;  // s13x
+                A[3] += c12 * ey;   // s12y
+                A[4] += c11 * ey;   // s11y
+                A[5] += c13d * ey;  // s13y
+                A[6] += c13d * ez;  // s13z
+                A[7] += c33 * ez;   // s33z
+                A[8] += csb * gz;   // sb_xz  (dz vx)
+                A[9] += csb * gx;   // sb_zx  (dx vz)
+                A[10] += csb * hy;  // sb_xy  (dy vx)
+                A[11] += csb * hx;  // sb_yx  (dx vy)
+                A[12] += csb * hz;  // sb_yz  (dz vy)
+                A[13] += csb * gy;  // sb_zy  (dy vz)
+            }
+            float fxx_x = 0.f, fxy_x = 0.f, fxz_x = 0.f, fzx = 0.f, fyx = 0.f;
+            float fyx_y = 0.f, fyy_y = 0.f, fyz_y = 0.f, fxy = 0.f, fzy = 0.f;
+            float fzx_z = 0.f, fzz_z = 0.f, fxz = 0.f, fyz = 0.f;
+            if (ix < pml || ix >= a.nx - pml) {
+                const int si = ix < pml ? ix : ix - (a.nx - 2 * pml);
+                const long long s = (long long)shot * a.slabx_stride + ((long long)si * a.ny + iy) * n + iz;
+                const float Ax = a.pax[ix], Bx = a.pbx[ix], Ah = a.paxh[ix], Bh = a.pbxh[ix];
+                float* p;
+                p = a.phx[0] + s; fxx_x = *p + dt * (-Ax * *p - Bx * A[0]); *p = fxx_x;
+                p = a.phx[1] + s; fxy_x = *p + dt * (-Ax * *p - Bx * A[1]); *p = fxy_x;
+                p = a.phx[2] + s; fxz_x = *p + dt * (-Ax * *p - Bx * A[2]); *p = fxz_x;
+                p = a.phx[3] + s; fzx = *p + dt * (-Ah * *p - Bh * A[9]); *p = fzx;
+                p = a.phx[4] + s; fyx = *p + dt * (-Ah * *p - Bh * A[11]); *p = fyx;
+            }
+            if (iy < pml || iy >= a.ny - pml) {
+                const int si = iy < pml ? iy : iy - (a.ny - 2 * pml);
+                const long long s = (long long)shot * a.slaby_stride + ((long long)ix * 2 * pml + si) * n + iz;
+                const float Ay = a.pay[iy], By = a.pby[iy], Ah = a.payh[iy], Bh = a.pbyh[iy];
+                float* p;
+                p = a.phy[0] + s; fyx_y = *p + dt * (-Ay * *p - By * A[3]); *p = fyx_y;
+                p = a.phy[1] + s; fyy_y = *p + dt * (-Ay * *p - By * A[4]); *p = fyy_y;
+                p = a.phy[2] + s; fyz_y = *p + dt * (-Ay * *p - By * A[5]); *p = fyz_y;
+                p = a.phy[3] + s; fxy = *p + dt * (-Ah * *p - Bh * A[10]); *p = fxy;
+                p = a.phy[4] + s; fzy = *p + dt * (-Ah * *p - Bh * A[13]); *p = fzy;
+            }
+            if (iz < pml || iz >= n - pml) {
+                const int si = iz < pml ? iz : iz - (n - 2 * pml);
+                const long long s = (long long)shot * a.slabz_stride + row * 2 * pml + si;
+                const float Az = a.paz[iz], Bz = a.pbz[iz], Ah = a.pazh[iz], Bh = a.pbzh[iz];
+                float* p;
+                p = a.phz[0] + s; fzx_z = *p + dt * (-Az * *p - Bz * A[6]); *p = fzx_z;
+                p = a.phz[1] + s; fzz_z = *p + dt * (-Az * *p - Bz * A[7]); *p = fzz_z;
+                p = a.phz[2] + s; fxz = *p + dt * (-Ah * *p - Bh * A[8]); *p = fxz;
+                p = a.phz[3] + s; fyz = *p + dt * (-Ah * *p - Bh * A[12]); *p = fyz;
+            }
+            float* pxx = a.sig[0] + g;
+            float* pyy = a.sig[1] + g;
+            float* pzz = a.sig[2] + g;
+            float* pxy = a.sig[3] + g;
+            float* pxz = a.sig[4] + g;
+            float* pyz = a.sig[5] + g;
+            *pxx = *pxx + dt * (A[0] + A[3] + A[6] + (fxx_x + fyx_y + fzx_z));
+            *pyy = *pyy + dt * (A[1] + A[4] + A[6] + (fxy_x + fyy_y + fzx_z));
+            *pzz = *pzz + dt * (A[2] + A[5] + A[7] + (fxz_x + fyz_y + fzz_z));
+            *pxz = *pxz + dt * (A[8] + fxz + A[9] + fzx);
+            *pyz = *pyz + dt * (A[12] + fyz + A[13] + fzy);
+            *pxy = *pxy + dt * (A[10] + fxy + A[11] + fyx);
+        }
+        if (ip == sip) {
+            // source after the stress update; this workgroup is the only one
+            // that reads this row of v (its z-derivative was staged above),
+            // so the write cannot race.
+            const float s = a.rik[it];
+            if (kind == 3) {
+                a.sig[0][g] += s;
+                a.sig[1][g] += s;
+                a.sig[2][g] += s;
+            } else if (kind >= 0 && kind < 3) {
+                float* vc = const_cast<float*>(a.fam[kind][1]);
+                vc[g] += s;
+            }
+        }
+    }
+}
+
+}  // namespace qvti

@@ -8,11 +8,17 @@ def small_case(nx, ny, nz, pml, nt, kind, vti, seed=1, dt=5e-4, dx=10.0, dy=12.0
     """Random heterogeneous model + shot dict in the oracle's format."""
     rng = np.random.default_rng(seed)
     shape = (nx, ny, nz)
-    rx = np.arange(1, nx - 1, 2)
+    sx, sy, sz = nx // 2, ny // 2 - 1, nz // 2 + 1
+    # line A: along x just below the top C-PML (far field, small amplitudes);
+    # line B: along y through the source column (near field)
+    rxa = np.arange(1, nx - 1, 2)
+    ryb = np.arange(0, ny)
+    rx = np.concatenate([rxa, np.full(len(ryb), sx + 1)])
+    ry = np.concatenate([np.full(len(rxa), ny // 2 + 1), ryb])
+    rz = np.concatenate([np.full(len(rxa), pml + 1), np.full(len(ryb), sz - 2)])
     shot = dict(nx=nx, ny=ny, nz=nz, pml=pml, nt=nt, dx=dx, dy=dy, dz=dz, dt=dt, f0=f0,
-                w0=float(np.float32(2 * PI * f0)), src_kind=kind,
-                sx=nx // 2, sy=ny // 2 - 1, sz=nz // 2 + 1,
-                rx=rx, ry=np.full(len(rx), ny // 2 + 1), rz=np.full(len(rx), pml + 1))
+                w0=float(np.float32(2 * PI * f0)), src_kind=kind, sx=sx, sy=sy, sz=sz,
+                rx=rx, ry=ry, rz=rz, line_a=len(rxa))
     t = np.arange(nt)
     shot["rik"] = (np.sin(t * 0.3) * np.exp(-0.01 * t)).astype(np.float32)
     vp = (2500 + 500 * rng.random(shape)).astype(np.float32)
