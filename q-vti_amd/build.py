@@ -24,8 +24,8 @@ def hipcc():
     return p if os.path.exists(p) else "hipcc"
 
 
-def build(verbose=False):
-    inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+def build(verbose=False, out=None, defines=()):
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC] + ["-D" + d for d in defines]
     common = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result"]
     objs = []
     bdir = os.path.join(HERE, "_build")
@@ -41,10 +41,11 @@ def build(verbose=False):
         cmd = [hipcc(), "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj] + common + inc
         _run(cmd, verbose)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    lib = out or LIB
+    tmp = lib + ".tmp"
     _run([hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"], verbose)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def _run(cmd, verbose):

@@ -283,6 +283,8 @@ extern "C" int qvti_device_count(int* n)
 
 extern "C" long long qvti_device_bytes(qvti_ctx* c) { return c ? c->bytes : -1; }
 
+hipStream_t qvti_ctx_stream(qvti_ctx* c) { return c ? c->stream : nullptr; }
+
 static int build_launch_config(qvti_ctx* c);
 
 extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, int device)
@@ -670,10 +672,10 @@ static void enqueue_step(qvti_ctx* c)
     };
     timed(KC_TICK, [&] { hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, c->d_it); });
     timed(KC_VEL_X, [&] {
-        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(c->g_vx.x, S), blk, 0, st, c->a_vx); });
+        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_VEL_FIRST>), dim3(c->g_vx.x, S), blk, 0, st, c->a_vx); });
     });
     timed(KC_VEL_Y, [&] {
-        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(c->g_vy.x, S), blk, 0, st, c->a_vy); });
+        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_VEL_ACC>), dim3(c->g_vy.x, S), blk, 0, st, c->a_vy); });
     });
     timed(KC_VEL_Z, [&] {
         dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_vel_z<decltype(g)>, dim3(c->g_vz.x, S), blk, 0, st, c->a_vz); });
@@ -691,10 +693,10 @@ static void enqueue_step(qvti_ctx* c)
         dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3(c->g_zr.x, S), blk, 0, st, c->a_zi); });
     });
     timed(KC_STR_X, [&] {
-        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(c->g_vx.x, S), blk, 0, st, c->a_sx); });
+        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_STORE>), dim3(c->g_vx.x, S), blk, 0, st, c->a_sx); });
     });
     timed(KC_STR_Y, [&] {
-        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(c->g_vy.x, S), blk, 0, st, c->a_sy); });
+        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_STORE>), dim3(c->g_vy.x, S), blk, 0, st, c->a_sy); });
     });
     timed(KC_STR_Z, [&] {
         dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_stress_z<decltype(g)>, dim3(c->g_sz.x, S), blk, 0, st, c->a_sz); });
@@ -1184,7 +1186,7 @@ extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym,
             a.job[0].sym = 0; a.job[0].half = 0;
             const int other = axis == 0 ? ny : nx;
             dispatch_n(n, [&](auto g) {
-                hipLaunchKernelGGL(k2_deriv_strided<decltype(g)>, dim3(other * a.ztiles, 1), dim3(kThreads), 0, 0, a);
+                hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_STORE>), dim3(other * a.ztiles, 1), dim3(kThreads), 0, 0, a);
             });
         } else {
             RowDerivArgs r{};

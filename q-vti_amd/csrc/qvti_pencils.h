@@ -33,6 +33,9 @@
 namespace qvti {
 
 constexpr int kThreads = 256;
+#ifndef QVTI_STRIDED_WAVES
+#define QVTI_STRIDED_WAVES 1
+#endif
 enum { MODE_STORE = 0, MODE_VEL_FIRST = 1, MODE_VEL_ACC = 2 };
 
 template <class G>
@@ -72,8 +75,9 @@ struct StridedArgs {
     DJob job[15];
 };
 
-template <class G>
-__global__ __launch_bounds__(kThreads) void k2_deriv_strided(StridedArgs a)
+template <class G, int MODE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_STRIDED_WAVES)))
+void k2_deriv_strided(StridedArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::N];
@@ -84,9 +88,9 @@ __global__ __launch_bounds__(kThreads) void k2_deriv_strided(StridedArgs a)
     const int nz = a.nz;
     const bool zok = zc < nz;
     const bool act1 = jj < G::N2, act2 = jj < G::N1;
-    const long long astride = a.axis == 0 ? a.sx : a.sy;
-    const long long obase = (a.axis == 0 ? (long long)other * a.sy : (long long)other * a.sx) + zc;
-    const long long soff = (long long)shot * a.shot_stride;
+    // uniform per-block base (SGPRs) + 32-bit per-thread element offsets
+    const int astride = (int)(a.axis == 0 ? a.sx : a.sy);
+    const long long ubase = (long long)shot * a.shot_stride + (a.axis == 0 ? (long long)other * a.sy : (long long)other * a.sx);
     float2* pen = lds + p * G::PS;
     load_twiddles<G>(tws, a.tw);
     for (int jb = 0; jb < a.njobs; ++jb) {
@@ -94,18 +98,19 @@ __global__ __launch_bounds__(kThreads) void k2_deriv_strided(StridedArgs a)
         __syncthreads();
         if (act1) {
             float2 v[G::N1];
-            const float* in = J.in + soff + obase;
+            const float* __restrict__ in = J.in + ubase;
 #pragma unroll
-            for (int j1 = 0; j1 < G::N1; ++j1)
-                v[j1] = zok ? *reinterpret_cast<const float2*>(in + (long long)(j1 * G::N2 + jj) * astride)
-                            : make_float2(0.f, 0.f);
+            for (int j1 = 0; j1 < G::N1; ++j1) {
+                const int off = (j1 * G::N2 + jj) * astride + zc;
+                v[j1] = zok ? *reinterpret_cast<const float2*>(in + off) : make_float2(0.f, 0.f);
+            }
             fwd_l1<G>(v, pen, tws, jj);
         }
         __syncthreads();
         float2 b[G::N2];
         if (act2) {
             fwd_l2<G>(b, pen, jj);
-            const float2* S = a.sym[J.sym];
+            const float2* __restrict__ S = a.sym[J.sym];
 #pragma unroll
             for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = c_mul(b[k2], S[jj + G::N1 * k2]);
         }
@@ -115,20 +120,20 @@ __global__ __launch_bounds__(kThreads) void k2_deriv_strided(StridedArgs a)
         if (act1 && zok) {
             float2 d[G::N1];
             inv_lb<G>(d, pen, jj);
+            float* __restrict__ out = J.out + ubase;
 #pragma unroll
             for (int j1 = 0; j1 < G::N1; ++j1) {
                 const int i = jj + G::N2 * j1;
-                const long long g = soff + obase + (long long)i * astride;
+                const int off = i * astride + zc;
                 float2 r = d[j1];
-                if (a.mode != MODE_STORE) {
+                if (MODE != MODE_STORE) {
                     const int ix = a.axis == 0 ? i : other;
                     const int iy = a.axis == 0 ? other : i;
                     const int pml = a.pml, n = G::N;
                     if ((i < pml || i >= n - pml) && ix < a.nx - 1 && iy < a.ny - 1) {
                         const int si = i < pml ? i : i - (n - 2 * pml);
-                        const long long sidx = a.axis == 0 ? ((long long)si * a.ny + iy) * nz + zc
-                                                           : ((long long)ix * 2 * pml + si) * nz + zc;
-                        float2* ph = reinterpret_cast<float2*>(J.phi + (long long)shot * a.slab_stride + sidx);
+                        const int sidx = a.axis == 0 ? (si * a.ny + iy) * nz + zc : (ix * 2 * pml + si) * nz + zc;
+                        float2* ph = reinterpret_cast<float2*>(J.phi + (long long)shot * a.slab_stride) + sidx / 2;
                         const float A = J.half ? a.pah[i] : a.pa[i];
                         const float B = J.half ? a.pbh[i] : a.pb[i];
                         float2 q = *ph;
@@ -136,12 +141,12 @@ __global__ __launch_bounds__(kThreads) void k2_deriv_strided(StridedArgs a)
                         if (zc + 1 < nz - 1) { q.y = q.y + a.dt * (-A * q.y - B * r.y); r.y = r.y + q.y; }
                         *ph = q;
                     }
-                    if (a.mode == MODE_VEL_ACC) {
-                        const float2 ac = *reinterpret_cast<const float2*>(J.acc + g);
+                    if (MODE == MODE_VEL_ACC) {
+                        const float2 ac = *reinterpret_cast<const float2*>(J.acc + ubase + off);
                         r = make_float2(ac.x + r.x, ac.y + r.y);
                     }
                 }
-                *reinterpret_cast<float2*>(J.out + g) = r;
+                *reinterpret_cast<float2*>(out + off) = r;
             }
         }
     }
@@ -670,12 +675,13 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
     const int it = *a.it;
     const int kind = a.src_kind[shot];
     const long long sip = a.src_ip[shot];
+#pragma unroll 1
     for (int cell = tid; cell < R * n; cell += kThreads) {
         const int rr = cell / n, iz = cell - rr * n;
         const long long row = r0 + rr;
         if (row >= a.nrows) continue;
         const int ix = (int)(row / a.ny), iy = (int)(row - (long long)ix * a.ny);
-        const long long ip = row * n + iz;
+        const int ip = (int)(row * n + iz);
         const long long g = soff + ip;
         auto S = [&](int f) { return st[(f * R + rr) * n + iz]; };
         if (ix < a.nx - 1 && iy < a.ny - 1 && iz < n - 1) {
@@ -689,8 +695,8 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
 #pragma unroll
             for (int jf = 0; jf < 5; ++jf) {
                 const float ez = S(fz[jf]), gz = S(fgx[jf]), hz = S(fhy[jf]);
-                const float ex = a.ex[jf][g], gx = a.gx[jf][g], hx = a.hx[jf][g];
-                const float ey = a.ey[jf][g], gy = a.gy[jf][g], hy = a.hy[jf][g];
+                const float ex = (a.ex[jf] + soff)[ip], gx = (a.gx[jf] + soff)[ip], hx = (a.hx[jf] + soff)[ip];
+                const float ey = (a.ey[jf] + soff)[ip], gy = (a.gy[jf] + soff)[ip], hy = (a.hy[jf] + soff)[ip];
                 const float c11 = a.c11[jf][ip], c33 = a.c33[jf][ip], c13 = a.c13[jf][ip];
                 const float cs = a.cs[jf][ip], csb = a.csb[jf][ip];
                 const float c12 = c11 - 2 * cs, c13d = c13 - 2 * cs;

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libqvti.so")
+LIB_PATH = os.environ.get("QVTI_LIB") or os.path.join(HERE, "libqvti.so")
 
 OK = 0
 E_ARG, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_STATE, E_UNSTABLE, E_BUDGET, E_CANCELLED, E_PENDING = range(-1, -10, -1)

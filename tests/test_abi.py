@@ -1,0 +1,99 @@
+"""CPU tests of the C ABI library (no GPU): it loads, exports every entry point
+include/qvti.h declares, fails loudly without a device, and its host helpers
+(viscoelas_3D.cpp's wavelet, gamma and receiver geometry; the seeded model
+generators) match their closed forms."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import qvti
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PI_T = 3.1415926
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "qvti.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qvti_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_entry_point():
+    L = qvti.lib()
+    names = header_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(qvti.EXPORTS)
+
+
+def test_no_device_fails_loudly():
+    if qvti.device_count() > 0:
+        pytest.skip("a device is visible")
+    c = ctypes.c_void_p()
+    g = qvti.Grid(16, 16, 16, 3, 10, 10, 10, 1e-3, 25, 157, 0)
+    rc = qvti.lib().qvti_create(ctypes.byref(c), ctypes.byref(g), 1, 0)
+    assert rc == qvti.E_DEVICE
+    assert b"device" in qvti.lib().qvti_last_error().lower()
+
+
+def test_unsupported_lengths_are_rejected_before_the_device():
+    c = ctypes.c_void_p()
+    for dims in ((18, 16, 16), (16, 16, 49), (535, 485, 48)):
+        g = qvti.Grid(*dims, 3, 10, 10, 10, 1e-3, 25, 157, 0)
+        rc = qvti.lib().qvti_create(ctypes.byref(c), ctypes.byref(g), 1, 0)
+        assert rc == qvti.E_UNSUPPORTED, dims
+        assert b"supported" in qvti.lib().qvti_last_error()
+
+
+def test_ricker_matches_reference_formula():
+    """viscoelas_3D.cpp:498-502: tmp = (pi f0 (it dt - 1/f0))^2 held in FP32."""
+    nt, dt, f0 = 400, np.float32(1e-3), np.float32(25.0)
+    r = qvti.ricker(nt, float(dt), float(f0))
+    it = np.arange(nt)
+    x = (it * dt).astype(np.float32) - np.float32(1) / f0
+    tmp = ((PI_T * float(f0)) * x.astype(np.float64)) ** 2
+    tmp = tmp.astype(np.float32)
+    ref = ((1.0 - 2.0 * tmp.astype(np.float64)) * np.exp(-tmp).astype(np.float32)).astype(np.float32)
+    assert np.allclose(r, ref, rtol=3e-7, atol=1e-12)  # exp implementations differ by <= 1 ulp
+    assert abs(r.max() - 1.0) < 1e-6 and np.argmax(r) == 40
+
+
+def test_gamma_and_receivers():
+    q = np.array([20, 50, 100, 200, 1e6], np.float32)
+    assert np.allclose(qvti.gamma_from_q(q), (np.arctan(1.0 / q.astype(np.float64)) / PI_T).astype(np.float32), rtol=0)
+    rx, ry = qvti.reference_receivers(15, 4, 3, 2)
+    ip = np.arange(12)
+    assert np.array_equal(rx, 15 + ip // 3 * 2) and np.array_equal(ry, 15 + ip % 3 * 2)
+
+
+def test_model_generators():
+    shape = (20, 16, 24)
+    h = qvti.make_model(0, 1, shape, [3000, 1730, 2200, 100, 50])
+    assert np.all(h["vp"] == 3000) and np.all(h["qs"] == 50) and np.all(h["rho"] == 2200)
+    lay = qvti.make_model(1, 7, shape, [1800, 4500, 0.25, 30, 200], nlay=6, vti=True)
+    assert len(np.unique(lay["vp"])) == 6
+    assert np.all(lay["vp"] == lay["vp"][:1, :1, :])          # flat in x and y
+    assert lay["vp"].min() >= 1800 and lay["vp"].max() <= 4500
+    assert np.allclose(lay["vs"], lay["vp"] / 1.73, rtol=1e-6)
+    assert np.allclose(lay["rho"], 310 * lay["vp"].astype(np.float64) ** 0.25, rtol=1e-6)
+    assert np.all(lay["delta"] <= lay["eps"]) and lay["eps"].max() <= 0.25
+    assert np.allclose(lay["qs"], lay["qp"] / 2)
+    sm = qvti.make_model(2, 3, shape, [2000, 5000, 0.25, 20, 150, 3, 0.05], vti=True)
+    assert abs(sm["vp"].min() - 2000) < 1e-3 and abs(sm["vp"].max() - 5000) < 1e-2
+    assert sm["qp"].min() >= 20 - 1e-4 and sm["qp"].max() <= 150 + 1e-3
+    assert np.allclose(sm["delta"], sm["eps"] / 2, rtol=1e-6)
+    # eps decreases as vp increases (co-varying fields)
+    assert sm["eps"].ravel()[np.argmax(sm["vp"])] < sm["eps"].ravel()[np.argmin(sm["vp"])]
+    again = qvti.make_model(2, 3, shape, [2000, 5000, 0.25, 20, 150, 3, 0.05], vti=True)
+    assert np.array_equal(again["vp"], sm["vp"])
+
+
+def test_estimate_bytes_scales_with_shots():
+    g = qvti.Grid(256, 256, 256, 20, 12.5, 12.5, 12.5, 1e-3, 20, 125.7, 1)
+    one = qvti.lib().qvti_estimate_bytes(ctypes.byref(g), 1)
+    four = qvti.lib().qvti_estimate_bytes(ctypes.byref(g), 4)
+    assert 3e9 < one < 12e9 and four > 2.5 * one
