@@ -131,6 +131,161 @@ def run_cpu_baseline(name, budget_s=25.0):
                       "1-step run (setup excluded)" % (name, steps)}
 
 
+def surface_shots(g, n, seed):
+    """n sources at seeded uniform (x, y) inside the C-PML frame, z = pml + 1,
+    each with the workload's receiver line."""
+    rng = np.random.default_rng(seed)
+    p = g["pml"]
+    xs = rng.integers(p, g["nx"] - p, n)
+    ys = rng.integers(p, g["ny"] - p, n)
+    rx = np.arange(g["nx"])
+    return [qvti.make_shot(qvti.SRC_FZ, (int(x), int(y), p + 1), rx, np.full(g["nx"], g["ny"] // 2),
+                           np.full(g["nx"], p + 1)) for x, y in zip(xs, ys)]
+
+
+def time_steps(e, nt, rik, shots, steps, warmup):
+    e.begin(nt, rik, shots)
+    e.step(warmup)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    return time.perf_counter() - t0
+
+
+def run_batched(steps, warmup, nshots=32, nt_full=2000, widths=(8, 16, 32)):
+    """32 shots of one 160x160x128 VTI model: the plain path (one shot per
+    launch sequence) against batched launches (W shots per kernel launch,
+    W picked by timing each candidate).  shots/s is for nt_full-step shots,
+    from the timed per-step wall time of each path."""
+    g, m, _ = make_workload("vti160")
+    shots = surface_shots(g, nshots, 11)
+    nt = warmup + steps
+    rik = qvti.ricker(nt, g["dt"], g["f0"])
+    with engine(g) as e:
+        e.set_model_dict(m)
+        e.set_probe(0)
+        plain_wall = time_steps(e, nt, rik, shots[:1], steps, warmup)
+    plain_step = plain_wall / steps
+    trials = {}
+    for w in widths:
+        with engine(g, max_shots=w) as e:
+            e.set_model_dict(m)
+            e.set_probe(0)
+            wall = time_steps(e, nt, rik, shots[:w], steps, warmup)
+        trials[w] = wall / steps / w  # seconds per shot-step
+    wbest = min(trials, key=trials.get)
+    per_shot_step = trials[wbest]
+    N = g["nx"] * g["ny"] * g["nz"]
+    plain_sps = 1.0 / (plain_step * nt_full)
+    batched_sps = 1.0 / (per_shot_step * nt_full)
+    return {
+        "metric": "batched shots/s (160x160x128 VTI, 2000-step shots)",
+        "value": round(batched_sps, 4), "unit": "shots/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(per_shot_step * wbest * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded smooth random VTI model)",
+        "config": {"workload": "32 shots on one 160x160x128 VTI model, pml 20, dx 12.5 m, dt 1 ms, sources "
+                               "seeded on the surface grid", "cells": N, "parallelism": "1 GPU, %d shots per launch" % wbest},
+        "plain_shots_per_s": round(plain_sps, 4),
+        "speedup_vs_plain": round(batched_sps / plain_sps, 3),
+        "gcell_steps_per_s": {"plain": round(N / plain_step / 1e9, 3), "batched": round(N / per_shot_step / 1e9, 3)},
+        "width_trials_ms_per_shot_step": {str(w): round(v * 1e3, 4) for w, v in trials.items()},
+    }
+
+
+SERVE_GRIDS = [(96, 96, 96), (128, 128, 128), (160, 160, 128), (200, 200, 100)]
+
+
+def serve_requests(n, seed=5, nt_range=(1000, 3000), rate=4.0):
+    """Seeded Poisson stream: grid from SERVE_GRIDS, isotropic or VTI with
+    probability 0.5, nt uniform in nt_range, Q in [20, 200], vp in
+    [2000, 4000], dx 12.5 m, dt 0.8 ms, 20-cell C-PML."""
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        nx, ny, nz = SERVE_GRIDS[rng.integers(len(SERVE_GRIDS))]
+        vti = bool(rng.random() < 0.5)
+        nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+        reqs.append(dict(t=t, shape=(nx, ny, nz), vti=vti, nt=nt, seed=int(rng.integers(1 << 30))))
+    return reqs
+
+
+def serve_inputs(r):
+    nx, ny, nz = r["shape"]
+    pml, d, dt, f0 = 20, 12.5, 8e-4, 20.0
+    m = qvti.make_model(2, r["seed"], (nx, ny, nz), [2000, 4000, 0.2, 20, 200, 8, 0.05], vti=r["vti"])
+    m["gp"] = qvti.gamma_from_q(m["qp"])
+    m["gs"] = qvti.gamma_from_q(m["qs"])
+    if not r["vti"]:
+        m.pop("eps", None)
+        m.pop("delta", None)
+    grid = qvti.Grid(nx, ny, nz, pml, d, d, d, dt, f0, float(np.float32(2 * PI * f0)), int(r["vti"]))
+    rx = np.arange(nx)
+    shot = qvti.make_shot(qvti.SRC_FZ, (nx // 2, ny // 2, pml + 1), rx, np.full(nx, ny // 2), np.full(nx, pml + 1))
+    rik = qvti.ricker(r["nt"], dt, f0)
+    return grid, m, shot, rik
+
+
+def run_served(nreq, max_active=4, slice_steps=50):
+    """Continuous batching against plain sequential execution of the same
+    seeded Poisson stream.  Sequential latencies come from each request's
+    measured plain service time in a FIFO single-server queue with the same
+    arrival times."""
+    reqs = serve_requests(nreq)
+    inputs = [serve_inputs(r) for r in reqs]
+    # plain sequential service times (context per request, as a caller of qvti_forward would)
+    service = []
+    for r, (grid, m, shot, rik) in zip(reqs, inputs):
+        t0 = time.perf_counter()
+        with qvti.Engine(grid.nx, grid.ny, grid.nz, grid.pml, grid.dx, grid.dy, grid.dz, grid.dt, grid.f0, grid.w0,
+                         vti=bool(grid.vti)) as e:
+            e.set_model_dict(m)
+            e.forward(r["nt"], rik, shot)
+        service.append(time.perf_counter() - t0)
+    free, seq_lat = 0.0, []
+    for r, s in zip(reqs, service):
+        start = max(free, r["t"])
+        free = start + s
+        seq_lat.append(free - r["t"])
+    seq_span = free - reqs[0]["t"]
+    # served
+    budget = 200 * (1 << 30)
+    lat = []
+    with qvti.Server(budget, max_active=max_active, steps_per_slice=slice_steps) as srv:
+        t0 = srv.clock()
+        ids = []
+        for r, (grid, m, shot, rik) in zip(reqs, inputs):
+            delay = t0 + r["t"] - srv.clock()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(grid, r["nt"], rik, m, shot))
+        for i in ids:
+            rc = srv.wait(i, 900.0)
+            if rc != 0:
+                raise RuntimeError("request %d ended with %d" % (i, rc))
+        infos = [srv.info(i)[1] for i in ids]
+        lat = [inf.t_done_s - inf.t_submit_s for inf in infos]
+        span = max(inf.t_done_s for inf in infos) - min(inf.t_submit_s for inf in infos)
+    cells = [int(np.prod(r["shape"])) * r["nt"] for r in reqs]
+    return {
+        "metric": "served shots/s and p50/p99 latency (seeded Poisson stream, 4 req/s offered)",
+        "value": round(nreq / span, 4), "unit": "shots/s", "n_gpus": 1, "steps": int(sum(r["nt"] for r in reqs)),
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded smooth random models)",
+        "config": {"workload": "%d requests, grids 96^3/128^3/160x160x128/200x200x100, iso or VTI, nt in "
+                               "[1000,3000], Q in [20,200], dx 12.5 m, dt 0.8 ms" % nreq,
+                   "parallelism": "1 GPU, up to %d requests in flight, %d-step slices" % (max_active, slice_steps)},
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "sequential": {"shots_per_s": round(nreq / seq_span, 4),
+                       "p50": round(float(np.percentile(seq_lat, 50)), 3),
+                       "p99": round(float(np.percentile(seq_lat, 99)), 3)},
+        "speedup_vs_sequential": round(seq_span / span, 3),
+        "offered_gcell_steps_per_s": round(sum(cells) / (reqs[-1]["t"] - reqs[0]["t"]) / 1e9, 3),
+        "served_gcell_steps_per_s": round(sum(cells) / span / 1e9, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,11 +293,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="vti256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched", action="store_true", help="32-shot 160x160x128 batched leg instead")
+    ap.add_argument("--served", type=int, default=0, metavar="NREQ", help="serving leg with NREQ requests instead")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (shots shard across GPUs with no communication; "
-              "see DESIGN.md); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py: this build runs on one GPU only (shots shard across GPUs with no communication); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.batched:
+        print(json.dumps(run_batched(args.steps, args.warmup)), flush=True)
+        return
+    if args.served:
+        print(json.dumps(run_served(args.served)), flush=True)
+        return
     name = args.workload
     g, wall, kt, setup_s = run_plain(name, args.steps, args.warmup)
     N = g["nx"] * g["ny"] * g["nz"]

@@ -610,9 +610,11 @@ extern "C" int qvti_set_model(qvti_ctx* c, const float* velp, const float* gama_
     HIPCHK(hipSetDevice(c->device));
     const long long N = c->N;
     const size_t B = sizeof(float) * N;
-    float* tmp = nullptr;
+    // The model is staged in the spectral scratch buffer (15 fields per shot,
+    // free between runs and ordered on the context's stream), so setting a
+    // model allocates nothing and never waits for other contexts' work.
+    float* tmp = c->bufA;
     const int nin = c->g.vti ? 7 : 5;
-    HIPCHK(hipMalloc(&tmp, B * nin));
     const float* srcs[7] = {velp, gama_p, vels, gama_s, rho, eps, delta};
     int rc = QVTI_OK;
     for (int k = 0; k < nin && !rc; ++k)
@@ -642,7 +644,6 @@ extern "C" int qvti_set_model(qvti_ctx* c, const float* velp, const float* gama_
         if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess)
             rc = qvti_fail(QVTI_E_DEVICE, "model setup kernels failed");
     }
-    hipFree(tmp);
     if (!rc) c->has_model = true;
     return rc;
 }

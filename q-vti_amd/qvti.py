@@ -331,3 +331,62 @@ class Engine:
             nm = names.raw[j * 32:(j + 1) * 32].split(b"\0")[0].decode()
             res[nm] = {"launches": int(launches[j]), "total_ms": float(ms[j]), "bytes_per_launch": float(byts[j])}
         return res
+
+
+class Server:
+    """Continuous-batching request server (qvti_server_*): requests of any
+    supported grid are admitted by HBM budget and stepped concurrently."""
+
+    def __init__(self, budget_bytes, max_active=4, steps_per_slice=25, device=0):
+        self._s = ctypes.c_void_p()
+        check(lib().qvti_server_create(ctypes.byref(self._s), device, int(budget_bytes), max_active,
+                                       steps_per_slice))
+        self._meta = {}
+
+    def submit(self, grid, nt, rik, model, shot):
+        """grid: Grid; model: dict vp, gp, vs, gs, rho (+ eps, delta when grid.vti)."""
+        keep = [f32(rik)] + [f32(model.get(k)) for k in ("vp", "gp", "vs", "gs", "rho", "eps", "delta")]
+        q = Request(grid, nt, *[_p(a) for a in keep[1:]], _p(keep[0]), shot)
+        rid = ctypes.c_uint64()
+        check(lib().qvti_server_submit(self._s, ctypes.byref(q), ctypes.byref(rid)))
+        self._meta[rid.value] = (nt, shot.nrec)
+        return rid.value
+
+    def wait(self, rid, timeout_s=600.0):
+        return lib().qvti_server_wait(self._s, rid, timeout_s)
+
+    def cancel(self, rid):
+        check(lib().qvti_server_cancel(self._s, rid))
+
+    def info(self, rid):
+        inf = ResultInfo()
+        rc = lib().qvti_server_poll(self._s, rid, ctypes.byref(inf), None, None, None)
+        return rc, inf
+
+    def result(self, rid):
+        """(status, info, records[3, nt, nrec] or None)."""
+        nt, nrec = self._meta[rid]
+        rec = np.zeros((3, nt, nrec), np.float32)
+        inf = ResultInfo()
+        rc = lib().qvti_server_poll(self._s, rid, ctypes.byref(inf), _p(rec[0]), _p(rec[1]), _p(rec[2]))
+        return rc, inf, (rec if rc == 0 else None)
+
+    def clock(self):
+        return lib().qvti_server_clock(self._s)
+
+    def close(self):
+        if self._s:
+            lib().qvti_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,120 @@
+"""GPU tests of the continuous-batching server (qvti_server_*).
+
+Serving must not change results: every request's seismograms are bit-identical
+whether it runs alone or mixed with requests of other grid sizes, durations
+and media (isotropic/VTI), and identical to the plain single-shot path
+(qvti_forward), which the parity tests compare with the oracle.  Admission by
+HBM budget, queueing when the budget is full, rejection of requests larger
+than the budget, and cancellation are checked here too.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import qvti
+from cases import rel_l2, small_case
+
+pytestmark = pytest.mark.gpu
+
+# nx, ny, nz, pml, nt, source kind, vti, seed
+REQS = [
+    (16, 20, 24, 3, 40, O.SRC_FX, False, 1),
+    (24, 20, 16, 4, 30, O.SRC_FZ, True, 2),
+    (20, 16, 32, 3, 55, O.SRC_EXPLOSIVE, False, 3),
+    (16, 20, 24, 3, 25, O.SRC_FZ, True, 4),
+    (20, 16, 32, 3, 35, O.SRC_FY, False, 5),   # same grid as the third: reuses a pooled context
+    (32, 24, 20, 5, 45, O.SRC_EXPLOSIVE, True, 6),
+]
+
+
+def build(req):
+    nx, ny, nz, pml, nt, kind, vti, seed = req
+    shot, model = small_case(nx, ny, nz, pml, nt, kind, vti, seed=seed)
+    grid = qvti.Grid(nx, ny, nz, pml, shot["dx"], shot["dy"], shot["dz"], shot["dt"], shot["f0"], shot["w0"],
+                     int(vti))
+    sh = qvti.make_shot(kind, (shot["sx"], shot["sy"], shot["sz"]), shot["rx"], shot["ry"], shot["rz"])
+    return grid, shot, model, sh
+
+
+def plain(req):
+    grid, shot, model, sh = build(req)
+    with qvti.Engine(grid.nx, grid.ny, grid.nz, grid.pml, grid.dx, grid.dy, grid.dz, grid.dt, grid.f0, grid.w0,
+                     vti=bool(grid.vti)) as e:
+        e.set_model_dict(model)
+        return e.forward(shot["nt"], shot["rik"], sh)
+
+
+def serve(reqs, budget, max_active, slice_steps):
+    out, infos = [], []
+    with qvti.Server(budget, max_active=max_active, steps_per_slice=slice_steps) as s:
+        ids = []
+        for r in reqs:
+            grid, shot, model, sh = build(r)
+            ids.append(s.submit(grid, shot["nt"], shot["rik"], model, sh))
+        for i in ids:
+            assert s.wait(i, 300.0) == 0, qvti.lib().qvti_last_error()
+            rc, inf, rec = s.result(i)
+            assert rc == 0
+            out.append(rec)
+            infos.append((inf.t_submit_s, inf.t_admit_s, inf.t_done_s))
+    return out, infos
+
+
+def estimate(req):
+    grid = build(req)[0]
+    return qvti.lib().qvti_estimate_bytes(grid, 1)
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix():
+    big = 1 << 40
+    alone = [serve([r], big, 1, 9)[0][0] for r in REQS]
+    mixed, infos = serve(REQS, big, 4, 7)
+    for r, a, m in zip(REQS, alone, mixed):
+        assert np.array_equal(a, m), r
+        assert np.array_equal(a, plain(r)), r
+    # the mix really overlapped: some request was admitted before another finished
+    assert any(infos[j][1] < infos[i][2] for i in range(len(infos)) for j in range(len(infos)) if i != j)
+
+
+def test_served_records_match_the_oracle():
+    r = REQS[1]
+    grid, shot, model, sh = build(r)
+    ref = O.run(shot, model, "f64")
+    rec, _ = serve([REQS[0], r, REQS[2]], 1 << 40, 3, 5)
+    la = shot["line_a"]
+    assert rel_l2(rec[1][:, :, la:], ref["records"][:, :, la:]) < 1e-5
+
+
+def test_budget_queues_and_rejects():
+    # room for exactly one context at a time: requests of different grids
+    # are admitted one after another, evicting the pooled context
+    reqs = [REQS[0], REQS[1], REQS[5]]
+    budget = int(max(estimate(r) for r in reqs) * 1.05)
+    out, infos = serve(reqs, budget, 4, 11)
+    for r, rec in zip(reqs, out):
+        assert np.array_equal(rec, plain(r)), r
+    order = sorted(infos, key=lambda t: t[1])
+    for a, b in zip(order, order[1:]):
+        assert b[1] >= a[2] - 1e-6  # admitted only after the previous one finished
+    # a request larger than the whole budget is rejected, the rest still run
+    with qvti.Server(budget, max_active=2) as s:
+        grid, shot, model, sh = build((64, 64, 64, 4, 10, O.SRC_FX, True, 7))
+        bad = s.submit(grid, shot["nt"], shot["rik"], model, sh)
+        grid, shot, model, sh = build(REQS[0])
+        good = s.submit(grid, shot["nt"], shot["rik"], model, sh)
+        assert s.wait(bad, 60.0) == qvti.E_BUDGET
+        assert s.wait(good, 60.0) == 0
+
+
+def test_cancel():
+    with qvti.Server(1 << 40, max_active=2, steps_per_slice=5) as s:
+        grid, shot, model, sh = build((32, 32, 32, 4, 4000, O.SRC_FX, False, 8))
+        rik = np.resize(shot["rik"], 4000) * 0  # a silent source: stable for any length
+        long_id = s.submit(grid, 4000, rik, model, sh)
+        grid, shot, model, sh = build(REQS[0])
+        short_id = s.submit(grid, shot["nt"], shot["rik"], model, sh)
+        s.cancel(long_id)
+        assert s.wait(long_id, 120.0) == qvti.E_CANCELLED
+        assert s.wait(short_id, 120.0) == 0
+        rc, _, rec = s.result(short_id)
+        assert rc == 0 and np.array_equal(rec, plain(REQS[0]))
