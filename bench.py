@@ -286,6 +286,20 @@ def run_served(nreq, max_active=4, slice_steps=50):
     }
 
 
+def copy_probe(mib=1024, reps=3):
+    """A plain device copy of known size (torch's elementwise copy kernel):
+    the reference point that scales hardware FETCH_SIZE / WRITE_SIZE counters
+    to bytes."""
+    import torch
+
+    x = torch.ones(mib << 18, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    return {"copy_probe_bytes_each_way": x.numel() * 4, "reps": reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,7 +309,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched", action="store_true", help="32-shot 160x160x128 batched leg instead")
     ap.add_argument("--served", type=int, default=0, metavar="NREQ", help="serving leg with NREQ requests instead")
+    ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        print(json.dumps(copy_probe()), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (shots shard across GPUs with no communication); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)

@@ -242,16 +242,36 @@ struct L2 {
     static constexpr int YS = N2 + 1, ZS = N1 + 1;
     static constexpr int PS0 = (N1 * YS > N2 * ZS ? N1 * YS : N2 * ZS);
     static constexpr int PS = PS0 | 1;
+    // Twiddle tables in LDS: T1[k1*N2 + j2] = W_N^(j2*k1) (forward level 1)
+    // and T2[ja*N1 + k1] = W_N^(ja*k1) (inverse level A, conjugated on use).
+    // Each thread reads its table with one base register and immediate
+    // offsets.  For N1 == N2 the two tables coincide.
+    static constexpr int T2OFF = N1 == N2 ? 0 : N;
+    static constexpr int TWN = N + T2OFF;
 };
 
+// Fill the twiddle tables (G::TWN entries) from tw[m] = W_N^m.
+template <class G>
+__host__ __device__ __forceinline__ void twiddle_entry(float2* t, const float2* tw, int idx)
+{
+    if (idx < G::N) {
+        const int k1 = idx / G::N2, j2 = idx - k1 * G::N2;
+        t[idx] = tw[j2 * k1];
+    } else {
+        const int r = idx - G::N, ja = r / G::N1, k1 = r - ja * G::N1;
+        t[idx] = tw[ja * k1];
+    }
+}
+
 // Forward level 1: a[j1] = x[j1*N2 + j2] -> DFT_N1 -> twiddle -> Y[p][k1][j2]
+// (tw: table T1)
 template <class G>
 __host__ __device__ __forceinline__ void fwd_l1(float2 (&a)[G::N1], float2* pen, const float2* tw, int j2)
 {
     rdft<G::N1, -1, 1>(a);
 #pragma unroll
     for (int k1 = 0; k1 < G::N1; ++k1) {
-        const float2 w = k1 ? tw[j2 * k1] : make_float2(1.f, 0.f);
+        const float2 w = k1 ? tw[k1 * G::N2 + j2] : make_float2(1.f, 0.f);
         pen[k1 * G::YS + j2] = k1 ? c_mul(a[k1], w) : a[k1];
     }
 }
@@ -266,13 +286,14 @@ __host__ __device__ __forceinline__ void fwd_l2(float2 (&b)[G::N2], const float2
 }
 
 // Inverse level A: IDFT_N2 over k2, twiddle conj(W_n^(ja*k1)) -> Z[p][ja][k1]
+// (tw: table T2)
 template <class G>
 __host__ __device__ __forceinline__ void inv_la(float2 (&b)[G::N2], float2* pen, const float2* tw, int k1)
 {
     rdft<G::N2, 1, 1>(b);
 #pragma unroll
     for (int ja = 0; ja < G::N2; ++ja) {
-        float2 w = tw[ja * k1];
+        float2 w = tw[ja * G::N1 + k1];
         w.y = -w.y;
         pen[ja * G::ZS + k1] = (ja && k1) ? c_mul(b[ja], w) : b[ja];
     }

@@ -36,12 +36,15 @@ constexpr int kThreads = 256;
 #ifndef QVTI_STRIDED_WAVES
 #define QVTI_STRIDED_WAVES 1
 #endif
+#ifndef QVTI_XPASS_WAVES
+#define QVTI_XPASS_WAVES 1
+#endif
 enum { MODE_STORE = 0, MODE_VEL_FIRST = 1, MODE_VEL_ACC = 2 };
 
 template <class G>
 __device__ __forceinline__ void load_twiddles(float2* tws, const float2* tw)
 {
-    for (int t = threadIdx.x; t < G::N; t += kThreads) tws[t] = tw[t];
+    for (int t = threadIdx.x; t < G::TWN; t += kThreads) twiddle_entry<G>(tws, tw, t);
 }
 
 // ------------------------------------------------------------------------
@@ -80,13 +83,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_S
 void k2_deriv_strided(StridedArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
-    __shared__ float2 tws[G::N];
+    __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
     const int other = blockIdx.x / a.ztiles;
     const int p = tid % G::P, jj = tid / G::P;
     const int zc = (blockIdx.x - other * a.ztiles) * 2 * G::P + 2 * p;
     const int nz = a.nz;
     const bool zok = zc < nz;
+    // columns past nz (last tile of a ragged z) load a valid column and are
+    // never stored, so the loads need no branch
+    const int zl = zok ? zc : nz - 2;
     const bool act1 = jj < G::N2, act2 = jj < G::N1;
     // uniform per-block base (SGPRs) + 32-bit per-thread element offsets
     const int astride = (int)(a.axis == 0 ? a.sx : a.sy);
@@ -98,11 +104,12 @@ void k2_deriv_strided(StridedArgs a)
         __syncthreads();
         if (act1) {
             float2 v[G::N1];
-            const float* __restrict__ in = J.in + ubase;
+            // uniform row pointer (SGPRs) + one unsigned 32-bit lane offset
+            const unsigned voff = (unsigned)(jj * astride + zl);
 #pragma unroll
             for (int j1 = 0; j1 < G::N1; ++j1) {
-                const int off = (j1 * G::N2 + jj) * astride + zc;
-                v[j1] = zok ? *reinterpret_cast<const float2*>(in + off) : make_float2(0.f, 0.f);
+                const float* __restrict__ rp = J.in + ubase + (long long)(j1 * G::N2) * astride;
+                v[j1] = *reinterpret_cast<const float2*>(rp + voff);
             }
             fwd_l1<G>(v, pen, tws, jj);
         }
@@ -115,21 +122,37 @@ void k2_deriv_strided(StridedArgs a)
             for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = c_mul(b[k2], S[jj + G::N1 * k2]);
         }
         __syncthreads();
-        if (act2) inv_la<G>(b, pen, tws, jj);
+        if (act2) inv_la<G>(b, pen, tws + G::T2OFF, jj);
         __syncthreads();
-        if (act1 && zok) {
-            float2 d[G::N1];
-            inv_lb<G>(d, pen, jj);
-            float* __restrict__ out = J.out + ubase;
+        const unsigned voff = (unsigned)(jj * astride + zc);
+        if (MODE == MODE_STORE) {
+            if (act1 && zok) {
+                float2 d[G::N1];
+                inv_lb<G>(d, pen, jj);
 #pragma unroll
-            for (int j1 = 0; j1 < G::N1; ++j1) {
-                const int i = jj + G::N2 * j1;
-                const int off = i * astride + zc;
-                float2 r = d[j1];
-                if (MODE != MODE_STORE) {
+                for (int j1 = 0; j1 < G::N1; ++j1) {
+                    const long long urow = ubase + (long long)(G::N2 * j1) * astride;
+                    *reinterpret_cast<float2*>(J.out + urow + voff) = d[j1];
+                }
+            }
+        } else {
+            // The C-PML branch differs per element: park the pencil's result in
+            // its own LDS slots (natural order) and finish element by element,
+            // which keeps the register footprint at the FFT's.
+            float2 d[G::N1];
+            if (act1 && zok) inv_lb<G>(d, pen, jj);
+            __syncthreads();
+            if (act1 && zok) {
+#pragma unroll
+                for (int j1 = 0; j1 < G::N1; ++j1) pen[jj + G::N2 * j1] = d[j1];
+                const int pml = a.pml, n = G::N;
+#pragma unroll 2
+                for (int j1 = 0; j1 < G::N1; ++j1) {
+                    const int i = jj + G::N2 * j1;
+                    const long long urow = ubase + (long long)(G::N2 * j1) * astride;
+                    float2 r = pen[i];
                     const int ix = a.axis == 0 ? i : other;
                     const int iy = a.axis == 0 ? other : i;
-                    const int pml = a.pml, n = G::N;
                     if ((i < pml || i >= n - pml) && ix < a.nx - 1 && iy < a.ny - 1) {
                         const int si = i < pml ? i : i - (n - 2 * pml);
                         const int sidx = a.axis == 0 ? (si * a.ny + iy) * nz + zc : (ix * 2 * pml + si) * nz + zc;
@@ -142,11 +165,11 @@ void k2_deriv_strided(StridedArgs a)
                         *ph = q;
                     }
                     if (MODE == MODE_VEL_ACC) {
-                        const float2 ac = *reinterpret_cast<const float2*>(J.acc + ubase + off);
+                        const float2 ac = *reinterpret_cast<const float2*>(J.acc + urow + voff);
                         r = make_float2(ac.x + r.x, ac.y + r.y);
                     }
+                    *reinterpret_cast<float2*>(J.out + urow + voff) = r;
                 }
-                *reinterpret_cast<float2*>(out + off) = r;
             }
         }
     }
@@ -185,16 +208,20 @@ template <class G>
 __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
-    __shared__ float2 tws[G::N];
+    __shared__ float2 tws[G::TWN];
     constexpr int n = G::N;
     const int tid = threadIdx.x, shot = blockIdx.y;
     // level 1/B mapping (column j fastest) and level 2/A mapping (row k1 fastest)
     const int j = tid % G::N2, p1 = tid / G::N2;
     const int k1 = tid % G::N1, p2 = tid / G::N1;
     const bool act1 = p1 < G::P, act2 = p2 < G::P;
-    const long long r1 = (long long)blockIdx.x * G::P + p1, r2 = (long long)blockIdx.x * G::P + p2;
+    const long long r0 = (long long)blockIdx.x * G::P;
+    const long long r1 = r0 + p1, r2 = r0 + p2;
     const bool ok1 = act1 && r1 < a.nrows, ok2 = act2 && r2 < a.nrows;
-    const long long soff = (long long)shot * a.shot_stride;
+    // uniform block base (shot + first row) and unsigned 32-bit lane offsets;
+    // rows past the end (ragged last block) read row r0 and store nothing
+    const long long ubase = (long long)shot * a.shot_stride + r0 * n;
+    const unsigned lrow1 = ok1 ? (unsigned)(p1 * n) : 0u;
     float2* pen1 = lds + p1 * G::PS;
     float2* pen2 = lds + p2 * G::PS;
     load_twiddles<G>(tws, a.tw);
@@ -206,8 +233,8 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
             float2 v[G::N1];
 #pragma unroll
             for (int j1 = 0; j1 < G::N1; ++j1) {
-                const long long g = soff + r1 * n + j1 * G::N2 + j;
-                v[j1] = ok1 ? make_float2(fa[g], fb ? fb[g] : 0.f) : make_float2(0.f, 0.f);
+                const unsigned o = lrow1 + j1 * G::N2 + j;
+                v[j1] = make_float2(fa[ubase + o], fb ? fb[ubase + o] : 0.f);
             }
             fwd_l1<G>(v, pen1, tws, j);
         }
@@ -219,13 +246,10 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
             for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = c_mul(b[k2], S[k1 + G::N1 * k2]);
         }
         __syncthreads();
-        if (act2) inv_la<G>(b, pen2, tws, k1);
+        if (act2) inv_la<G>(b, pen2, tws + G::T2OFF, k1);
         __syncthreads();
         if (act1) inv_lb<G>(d, pen1, j);
     };
-    float2 dA[G::N1], dB[G::N1];
-    deriv(a.pxz, a.pyz, a.symp, dA);
-    deriv(a.pzz, nullptr, a.symm, dB);
 
     const int it = *a.it;
     const int kind = a.src_kind[shot];
@@ -233,52 +257,59 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
     const float sprev = it > 0 ? a.rik[it - 1] : 0.f;
     const int ix = (int)(r1 / a.ny), iy = (int)(r1 - (long long)ix * a.ny);
     const bool rowdom = ok1 && ix < a.nx - 1 && iy < a.ny - 1;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    const long long slab_row = (long long)shot * a.slab_stride + r1 * 2 * a.pml;
+
+    // velocity update of component c from its z-derivative (part of d), then
+    // the forward z-FFT of v + i dv into P[c]
+    auto update = [&](int c, const float2 (&d)[G::N1], bool imag) {
         float2 x[G::N1];
+        float* __restrict__ vc = a.v[c] + ubase;
+        const float* __restrict__ ac = a.acc[c] + ubase;
+        const float* __restrict__ bu = a.buoy[c] + (ubase - (long long)shot * a.shot_stride);
 #pragma unroll
         for (int j1 = 0; j1 < G::N1; ++j1) {
             const int iz = j + G::N2 * j1;
-            const long long ip = r1 * n + iz;
-            float d = c == 0 ? dA[j1].x : (c == 1 ? dA[j1].y : dB[j1].x);
-            float vold = 0.f, vnew = 0.f;
-            if (ok1) {
-                vold = a.v[c][soff + ip];
-                vnew = vold;
-                if (rowdom && iz < n - 1) {
-                    if (iz < a.pml || iz >= n - a.pml) {
-                        const int si = iz < a.pml ? iz : iz - (n - 2 * a.pml);
-                        float* ph = a.phi[c] + (long long)shot * a.slab_stride + r1 * 2 * a.pml + si;
-                        const bool h = c == 2;
-                        const float A = h ? a.pah[iz] : a.pa[iz];
-                        const float B = h ? a.pbh[iz] : a.pb[iz];
-                        const float q = *ph + a.dt * (-A * *ph - B * d);
-                        *ph = q;
-                        d = d + q;
-                    }
-                    vnew = vold + a.buoy[c][ip] * (a.acc[c][soff + ip] + d);
-                    a.v[c][soff + ip] = vnew;
+            const unsigned o = lrow1 + iz;
+            float dz = imag ? d[j1].y : d[j1].x;
+            const float vold = vc[o];
+            float vnew = vold;
+            if (rowdom && iz < n - 1) {
+                if (iz < a.pml || iz >= n - a.pml) {
+                    const int si = iz < a.pml ? iz : iz - (n - 2 * a.pml);
+                    float* ph = a.phi[c] + slab_row + si;
+                    const bool h = c == 2;
+                    const float A = h ? a.pah[iz] : a.pa[iz];
+                    const float B = h ? a.pbh[iz] : a.pb[iz];
+                    const float q = *ph + a.dt * (-A * *ph - B * dz);
+                    *ph = q;
+                    dz = dz + q;
                 }
-                float dv = vnew - vold;
-                if (ip == sip && kind == c) dv += sprev;
-                a.dv[c][soff + ip] = dv;
-                x[j1] = make_float2(vnew, dv);
-            } else {
-                x[j1] = make_float2(0.f, 0.f);
+                vnew = vold + bu[o] * (ac[o] + dz);
+                if (ok1) vc[o] = vnew;
             }
+            float dv = vnew - vold;
+            if (r1 * n + iz == sip && kind == c) dv += sprev;
+            if (ok1) a.dv[c][ubase + o] = dv;
+            x[j1] = make_float2(vnew, dv);
         }
-        // forward z-FFT of v + i dv (first pass of the 3-D transform)
         __syncthreads();
         if (act1) fwd_l1<G>(x, pen1, tws, j);
         __syncthreads();
         if (ok2) {
             float2 b[G::N2];
             fwd_l2<G>(b, pen2, k1);
-            float2* dst = a.P[c] + soff + r2 * n;
+            float2* dst = a.P[c] + ubase + (unsigned)(p2 * n);
 #pragma unroll
             for (int k2 = 0; k2 < G::N2; ++k2) dst[k1 + G::N1 * k2] = b[k2];
         }
-    }
+    };
+
+    float2 d[G::N1];
+    deriv(a.pxz, a.pyz, a.symp, d);  // dz pxz (vx) + i dz pyz (vy), D+
+    update(0, d, false);
+    update(1, d, true);
+    deriv(a.pzz, nullptr, a.symm, d);  // dz pzz (vz), D-
+    update(2, d, false);
 }
 
 // ------------------------------------------------------------------------
@@ -300,7 +331,7 @@ template <class G, int DIR>
 __global__ __launch_bounds__(kThreads) void k2_cfft_strided(CStridedArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
-    __shared__ float2 tws[G::N];
+    __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
     const int other = blockIdx.x / a.ztiles;
     const int p = tid % G::P, jj = tid / G::P;
@@ -334,7 +365,7 @@ __global__ __launch_bounds__(kThreads) void k2_cfft_strided(CStridedArgs a)
                 float2 b[G::N2];
 #pragma unroll
                 for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = base[(long long)(jj + G::N1 * k2) * astride];
-                inv_la<G>(b, pen, tws, jj);
+                inv_la<G>(b, pen, tws + G::T2OFF, jj);
             }
             __syncthreads();
             if (act1) {
@@ -368,10 +399,10 @@ struct XPassArgs {
 };
 
 template <class G>
-__global__ __launch_bounds__(kThreads) void k2_xpass(XPassArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS_WAVES))) void k2_xpass(XPassArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
-    __shared__ float2 tws[G::N];
+    __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
     const int iy = blockIdx.x / a.ztiles;
     const int p = tid % G::P, jj = tid / G::P;
@@ -406,13 +437,21 @@ __global__ __launch_bounds__(kThreads) void k2_xpass(XPassArgs a)
                 for (int k2 = 0; k2 < G::N2; ++k2) {
                     const double kx = a.kfx[jj + G::N1 * k2];
                     const double k2s = kx * kx + kyz;
-                    double s;
-                    if (op == 0) s = k2s > 0.0 ? a.invN / sqrt(k2s) : 0.0;
-                    else s = a.invN * sqrt(k2s);
+                    // 1/sqrt(k2s) from the hardware estimate and two Newton
+                    // steps (relative error far below 2^-50, so the FP32
+                    // rounding below matches that of the exact FP64 value)
+                    double y = __builtin_amdgcn_rsq(k2s);
+                    const double h = 0.5 * k2s;
+                    y = fma(y, fma(-h * y, y, 0.5), y);
+                    y = fma(y, fma(-h * y, y, 0.5), y);
+                    // (at k = 0 the estimate is inf and y is NaN: both
+                    // symbols are 0 there, |k|^-1 by the convention above)
+                    double s = 0.0;
+                    if (k2s > 0.0) s = op == 0 ? a.invN * y : a.invN * (k2s * y);
                     const float fs = (float)s;
                     c[k2] = make_float2(b[k2].x * fs, b[k2].y * fs);
                 }
-                inv_la<G>(c, pen, tws, jj);
+                inv_la<G>(c, pen, tws + G::T2OFF, jj);
             }
             __syncthreads();
             if (act1) {
@@ -445,7 +484,7 @@ template <class G>
 __global__ __launch_bounds__(kThreads) void k2_zrows(ZRowsArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
-    __shared__ float2 tws[G::N];
+    __shared__ float2 tws[G::TWN];
     constexpr int n = G::N;
     const int tid = threadIdx.x, shot = blockIdx.y;
     const int j = tid % G::N2, p1 = tid / G::N2;
@@ -480,7 +519,7 @@ __global__ __launch_bounds__(kThreads) void k2_zrows(ZRowsArgs a)
                 const float2* src = a.cin[f] + soff + r2 * n;
 #pragma unroll
                 for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = src[k1 + G::N1 * k2];
-                inv_la<G>(b, pen2, tws, k1);
+                inv_la<G>(b, pen2, tws + G::T2OFF, k1);
             }
             __syncthreads();
             if (ok1) {
@@ -519,7 +558,7 @@ template <class G>
 __global__ __launch_bounds__(kThreads) void k2_deriv_rows(RowDerivArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
-    __shared__ float2 tws[G::N];
+    __shared__ float2 tws[G::TWN];
     constexpr int n = G::N;
     const int tid = threadIdx.x;
     const int j = tid % G::N2, p1 = tid / G::N2;
@@ -547,7 +586,7 @@ __global__ __launch_bounds__(kThreads) void k2_deriv_rows(RowDerivArgs a)
         for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = c_mul(b[k2], a.sym[k1 + G::N1 * k2]);
     }
     __syncthreads();
-    if (act2) inv_la<G>(b, pen2, tws, k1);
+    if (act2) inv_la<G>(b, pen2, tws + G::T2OFF, k1);
     __syncthreads();
     if (act1) {
         float2 d[G::N1];
@@ -605,7 +644,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
     static_assert(R >= 1, "tile too small");
     static_assert(G::PS >= n, "staging must fit in the pencil buffer");
     __shared__ float2 lds[8 * R * G::PS];
-    __shared__ float2 tws[G::N];
+    __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
     const int j = tid % G::N2, q1 = tid / G::N2;
     const int k1 = tid % G::N1, q2 = tid / G::N1;
@@ -652,7 +691,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
             for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = c_mul(b[k2], S[k1 + G::N1 * k2]);
         }
         __syncthreads();
-        if (act2) inv_la<G>(b, pen2, tws, k1);
+        if (act2) inv_la<G>(b, pen2, tws + G::T2OFF, k1);
         __syncthreads();
         if (act1) inv_lb<G>(d, pen1, j);
         __syncthreads();
