@@ -7,11 +7,13 @@ steps then K timed steps.  Prints ONE JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload vti256|vti400|vti128|iso64]
                   [--batched] [--served] [--no-cpu-baseline]
 
-value = total cells x K / timed wall seconds / 1e9 (Gcell-steps/s).  The
-timed region brackets K steps with a device synchronisation on both sides;
-every kernel launch inside it is bracketed by HIP events on the engine's
-stream, from which the dominant kernel's average duration and achieved
-algorithmic GB/s are computed.
+value = total cells x K / timed wall seconds / 1e9 (Gcell-steps/s) over K
+steps of the production path (HIP graph per step), bracketed by device
+synchronisations.  A second region of K steps launches the same kernels
+directly with every launch bracketed by HIP events on the engine's stream;
+the dominant kernel's average duration there gives the achieved algorithmic
+GB/s of the roofline object.  roofline.traffic is the measured HBM bytes per
+launch of that kernel from the committed rocprofv3 counter summary.
 """
 import argparse
 import json
@@ -71,27 +73,47 @@ def engine(g, max_shots=1):
 
 
 def run_plain(name, steps, warmup):
+    """Two timed regions of `steps` steps each on one shot, after `warmup`
+    steps: (1) the production path (one HIP graph per step, blow-up probe
+    every 50 steps) gives the throughput; (2) the same kernels launched
+    directly, each bracketed by HIP events on the engine's stream, gives the
+    per-kernel average durations."""
     g, m, sh = make_workload(name)
-    nt = warmup + steps
+    nt = warmup + 2 * steps
     rik = qvti.ricker(nt, g["dt"], g["f0"])
     e = engine(g)
     t0 = time.time()
     e.set_model_dict(m)
     setup_s = time.time() - t0
-    e.set_graphs(False)
-    e.set_probe(0)
     e.begin(nt, rik, [qvti.make_shot(sh["kind"], sh["src"], sh["rx"], sh["ry"], sh["rz"])])
     e.step(warmup)
-    e.sync()
-    e.enable_kernel_timing(steps)
     e.sync()
     t0 = time.perf_counter()
     e.step(steps)
     e.sync()
     wall = time.perf_counter() - t0
+    e.enable_kernel_timing(steps)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    wall_timed = time.perf_counter() - t0
     kt = e.kernel_times()
     e.close()
-    return g, wall, kt, setup_s
+    return g, wall, wall_timed, kt, setup_s
+
+
+def measured_traffic(name, kernel):
+    """HBM bytes per launch of `kernel` on workload `name` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy probe), or
+    None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "%s_traffic.json" % name)
+    try:
+        d = json.load(open(path))
+        k = d["kernels"][kernel]
+        return round(k["fetch_bytes"] + k["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def run_cpu_baseline(name, budget_s=25.0):
@@ -325,7 +347,7 @@ def main():
         print(json.dumps(run_served(args.served)), flush=True)
         return
     name = args.workload
-    g, wall, kt, setup_s = run_plain(name, args.steps, args.warmup)
+    g, wall, wall_timed, kt, setup_s = run_plain(name, args.steps, args.warmup)
     N = g["nx"] * g["ny"] * g["nz"]
     gcs = N * args.steps / wall / 1e9
     balg = b_alg(g["nx"], g["ny"], g["nz"], g["pml"], g["vti"])
@@ -356,11 +378,14 @@ def main():
                                   "VTI" if g["vti"] else "isotropic"),
                    "cells": N, "parallelism": "1 GPU, 1 shot"},
         "roofline": {"bound": "hbm", "kernel": best, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes": round(k["bytes_per_launch"]), "avg_launch_us": round(avg_ms * 1e3, 2),
+                     "traffic": measured_traffic(name, best)},
         "step_roofline": {"b_alg_bytes_per_cell_step": round(balg, 1),
                           "achieved_GBps": round(balg * gcs, 1),
                           "frac": round(balg * gcs / HBM_PEAK_GBS, 4)},
         "kernels": per_kernel,
+        "ms_per_step_event_timed": round(wall_timed / args.steps * 1e3, 4),
         "setup_s": round(setup_s, 3),
     }
     if not args.no_cpu_baseline:

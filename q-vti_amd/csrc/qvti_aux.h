@@ -85,12 +85,27 @@ __device__ __forceinline__ void q_coeffs(double rho, double v, double g, double 
     o[4] = rho * (c * c) * (g * g) * pi / w0 / dt;
 }
 
+// Blocked field groups.  A group of nf per-cell fields is stored as 64-cell
+// blocks with the fields interleaved: element (f, ip) sits at
+// ((ip >> 6) * nf + f) * 64 + (ip & 63).  A thread reaches every field of its
+// cell from one address with immediate offsets f * 256 bytes, and the fields
+// of 64 neighbouring cells form one contiguous run of nf * 256 bytes.
+// N is always a multiple of 64 (every supported axis length is a multiple of 4).
+constexpr int kBlk = 64;
+__host__ __device__ __forceinline__ long long blk_index(long long ip, int nf, int f)
+{
+    return ((ip >> 6) * nf + f) * kBlk + (ip & 63);
+}
+// coefficient groups: G1 = {P 0-4, S 5-9, S-bar 10-14}, G2 = {H 0-4, D 5-9} (VTI)
+constexpr int kG1 = 15, kG2 = 10;
+
 struct CoefArgs {
     long long N;
     int vti;
     double w0, dt;
     const float *vp, *gp, *vs, *gs, *rho, *eps, *del;
-    float* out;  // [nsets*5][N]: P, S, (H, D)
+    float* g1;  // blocked [N/64][15][64]: P, S (S-bar filled by k_averages)
+    float* g2;  // blocked [N/64][10][64]: H, D (VTI only)
 };
 
 __global__ __launch_bounds__(256) void k_coefficients(CoefArgs a)
@@ -99,14 +114,14 @@ __global__ __launch_bounds__(256) void k_coefficients(CoefArgs a)
         double o[5];
         const double r = a.rho[ip];
         q_coeffs(r, a.vp[ip], a.gp[ip], a.w0, a.dt, o);
-        for (int j = 0; j < 5; ++j) a.out[(0 * 5 + j) * a.N + ip] = (float)o[j];
+        for (int j = 0; j < 5; ++j) a.g1[blk_index(ip, kG1, j)] = (float)o[j];
         q_coeffs(r, a.vs[ip], a.gs[ip], a.w0, a.dt, o);
-        for (int j = 0; j < 5; ++j) a.out[(1 * 5 + j) * a.N + ip] = (float)o[j];
+        for (int j = 0; j < 5; ++j) a.g1[blk_index(ip, kG1, 5 + j)] = (float)o[j];
         if (a.vti) {
             const double e = a.eps[ip], dl = a.del[ip], vp = a.vp[ip], vs = a.vs[ip];
             const double vh = e == 0.0 ? vp : vp * sqrt(1.0 + 2.0 * e);
             q_coeffs(r, vh, a.gp[ip], a.w0, a.dt, o);
-            for (int j = 0; j < 5; ++j) a.out[(2 * 5 + j) * a.N + ip] = (float)o[j];
+            for (int j = 0; j < 5; ++j) a.g2[blk_index(ip, kG2, j)] = (float)o[j];
             double vd = vp;
             if (dl != 0.0) {
                 const double c33 = r * vp * vp, c44 = r * vs * vs;
@@ -114,7 +129,7 @@ __global__ __launch_bounds__(256) void k_coefficients(CoefArgs a)
                 vd = sqrt((sqrt(prod > 0.0 ? prod : 0.0) + c44) / r);
             }
             q_coeffs(r, vd, a.gp[ip], a.w0, a.dt, o);
-            for (int j = 0; j < 5; ++j) a.out[(3 * 5 + j) * a.N + ip] = (float)o[j];
+            for (int j = 0; j < 5; ++j) a.g2[blk_index(ip, kG2, 5 + j)] = (float)o[j];
         }
     }
 }
@@ -125,8 +140,7 @@ struct AvgArgs {
     int nx, ny, nz;
     long long sx, sy, N;
     float dt;
-    const float* S;  // [5][N]
-    float* Sb;       // [5][N]
+    float* g1;       // coefficient group 1: reads S (fields 5-9), writes S-bar (10-14)
     const float* rho;
     float* buoy;     // [3][N]
 };
@@ -139,14 +153,14 @@ __global__ __launch_bounds__(256) void k_averages(AvgArgs a)
         const int iz = (int)(ip - (long long)ix * a.sx - (long long)iy * a.sy);
         const bool in = ix < a.nx - 1 && iy < a.ny - 1 && iz < a.nz - 1;
         for (int j = 0; j < 5; ++j) {
-            const float* c = a.S + (long long)j * a.N;
+            auto c = [&](long long q) { return a.g1[blk_index(q, kG1, 5 + j)]; };
             float v = 0.f;
             if (in) {
-                const float s8 = c[ip] + c[ip + 1] + c[ip + a.sy] + c[ip + 1 + a.sy] + c[ip + a.sx] + c[ip + a.sx + 1]
-                               + c[ip + a.sx + a.sy] + c[ip + a.sx + 1 + a.sy];
+                const float s8 = c(ip) + c(ip + 1) + c(ip + a.sy) + c(ip + 1 + a.sy) + c(ip + a.sx) + c(ip + a.sx + 1)
+                               + c(ip + a.sx + a.sy) + c(ip + a.sx + 1 + a.sy);
                 v = 0.125f * s8;
             }
-            a.Sb[(long long)j * a.N + ip] = v;
+            a.g1[blk_index(ip, kG1, 10 + j)] = v;
         }
         // rho + rho' is an FP32 sum in the reference, the rest FP64
         const float r = a.rho[ip];

@@ -169,8 +169,7 @@ struct qvti_ctx {
     int S = 1;  // max shots
     AxisDev ax[3];
     // model
-    float* coef = nullptr;  // sets P, S, (H, D): [nsets*5][N]
-    float* sbar = nullptr;  // [5][N]
+    float* coef = nullptr;  // blocked coefficient groups G1 (15 fields) then G2 (10, VTI)
     float* buoy = nullptr;  // [3][N]
     bool has_model = false;
     // state, field-major [field][shot][N]
@@ -344,9 +343,7 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
     c->slab[0] = (long long)p2 * g->ny * g->nz;
     c->slab[1] = (long long)g->nx * p2 * g->nz;
     c->slab[2] = (long long)g->nx * g->ny * p2;
-    const int nsets = g->vti ? 4 : 2;
-    rc = rc ? rc : dalloc(c, (void**)&c->coef, sizeof(float) * nsets * 5 * N);
-    rc = rc ? rc : dalloc(c, (void**)&c->sbar, sizeof(float) * 5 * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->coef, sizeof(float) * (kG1 + (g->vti ? kG2 : 0)) * N);
     rc = rc ? rc : dalloc(c, (void**)&c->buoy, sizeof(float) * 3 * N);
     rc = rc ? rc : dalloc(c, (void**)&c->v, sizeof(float) * 3 * S * N);
     rc = rc ? rc : dalloc(c, (void**)&c->sig, sizeof(float) * 6 * S * N);
@@ -533,13 +530,16 @@ static int build_launch_config(qvti_ctx* c)
     for (int k = 0; k < 3; ++k) {
         fam[k][0] = H1v[k]; fam[k][1] = vv[k]; fam[k][2] = H3v[k]; fam[k][3] = H1dv[k]; fam[k][4] = dvv[k];
     }
+    // x/y derivatives of the stress step: blocked groups of 15 fields per
+    // shot in bufA (x) and bufB (y); job k writes field k of the group
     float* DX[15];
     float* DY[15];
-    for (int k = 0; k < 15; ++k) { DX[k] = fld(c->bufA, k, c); DY[k] = fld(c->bufB, k, c); }
+    for (int k = 0; k < 15; ++k) { DX[k] = c->bufA + (long long)k * kBlk; DY[k] = c->bufB + (long long)k * kBlk; }
     StridedArgs& sxa = c->a_sx;
     strided_common(sxa, 0, Px);
     sxa.mode = MODE_STORE;
     sxa.njobs = 15;
+    sxa.bnf = kG1;
     for (int j = 0; j < 5; ++j) {
         sxa.job[j] = job(fam[0][j], DX[j], nullptr, nullptr, 0, 0);          // ex: dx vx-family, D+
         sxa.job[5 + j] = job(fam[2][j], DX[5 + j], nullptr, nullptr, 1, 0);   // gx: dx vz-family, D-
@@ -549,6 +549,7 @@ static int build_launch_config(qvti_ctx* c)
     strided_common(sya, 1, Py);
     sya.mode = MODE_STORE;
     sya.njobs = 15;
+    sya.bnf = kG1;
     for (int j = 0; j < 5; ++j) {
         sya.job[j] = job(fam[1][j], DY[j], nullptr, nullptr, 0, 0);          // ey: dy vy-family, D+
         sya.job[5 + j] = job(fam[2][j], DY[5 + j], nullptr, nullptr, 1, 0);   // gy: dy vz-family, D-
@@ -568,16 +569,10 @@ static int build_launch_config(qvti_ctx* c)
     sz.paz = Z.a; sz.pbz = Z.b; sz.pazh = Z.ah; sz.pbzh = Z.bh;
     for (int k = 0; k < 3; ++k)
         for (int j = 0; j < 5; ++j) sz.fam[k][j] = fam[k][j];
-    for (int j = 0; j < 5; ++j) {
-        sz.ex[j] = DX[j]; sz.gx[j] = DX[5 + j]; sz.hx[j] = DX[10 + j];
-        sz.ey[j] = DY[j]; sz.gy[j] = DY[5 + j]; sz.hy[j] = DY[10 + j];
-        const float* Pset = c->coef + (long long)(0 * 5 + j) * N;
-        const float* Sset = c->coef + (long long)(1 * 5 + j) * N;
-        const float* Hset = g.vti ? c->coef + (long long)(2 * 5 + j) * N : Pset;
-        const float* Dset = g.vti ? c->coef + (long long)(3 * 5 + j) * N : Pset;
-        sz.c11[j] = Hset; sz.c33[j] = Pset; sz.c13[j] = Dset;
-        sz.cs[j] = Sset; sz.csb[j] = c->sbar + (long long)j * N;
-    }
+    sz.dxb = c->bufA;
+    sz.dyb = c->bufB;
+    sz.g1 = c->coef;
+    sz.g2 = g.vti ? c->coef + (long long)kG1 * N : nullptr;
     for (int k = 0; k < 6; ++k) sz.sig[k] = sg[k];
     for (int k = 0; k < 5; ++k) {
         sz.phx[k] = c->phis[0] + (long long)k * c->S * c->slab[0];
@@ -629,15 +624,15 @@ extern "C" int qvti_set_model(qvti_ctx* c, const float* velp, const float* gama_
         ca.vp = tmp; ca.gp = tmp + N; ca.vs = tmp + 2 * N; ca.gs = tmp + 3 * N; ca.rho = tmp + 4 * N;
         ca.eps = c->g.vti ? tmp + 5 * N : nullptr;
         ca.del = c->g.vti ? tmp + 6 * N : nullptr;
-        ca.out = c->coef;
+        ca.g1 = c->coef;
+        ca.g2 = c->g.vti ? c->coef + (long long)kG1 * N : nullptr;
         const unsigned nb = (unsigned)std::min<long long>((N + kThreads - 1) / kThreads, 65535LL * 4);
         hipLaunchKernelGGL(k_coefficients, dim3(nb), dim3(kThreads), 0, c->stream, ca);
         AvgArgs aa;
         aa.nx = c->g.nx; aa.ny = c->g.ny; aa.nz = c->g.nz;
         aa.sx = (long long)c->g.ny * c->g.nz; aa.sy = c->g.nz; aa.N = N;
         aa.dt = c->g.dt;
-        aa.S = c->coef + 5 * N;
-        aa.Sb = c->sbar;
+        aa.g1 = c->coef;
         aa.rho = tmp + 4 * N;
         aa.buoy = c->buoy;
         hipLaunchKernelGGL(k_averages, dim3(nb), dim3(kThreads), 0, c->stream, aa);

@@ -75,6 +75,10 @@ struct StridedArgs {
     const float2* sym[2];
     const float *pa, *pb, *pah, *pbh;
     int njobs;
+    // STORE mode output layout: 0 = natural [shot][N] per job; nf > 0 =
+    // blocked group of nf fields per shot ([shot][N/64][nf][64], see
+    // blk_index), each job's out pointing at its field (group base + f*64)
+    int bnf;
     DJob job[15];
 };
 
@@ -96,7 +100,8 @@ void k2_deriv_strided(StridedArgs a)
     const bool act1 = jj < G::N2, act2 = jj < G::N1;
     // uniform per-block base (SGPRs) + 32-bit per-thread element offsets
     const int astride = (int)(a.axis == 0 ? a.sx : a.sy);
-    const long long ubase = (long long)shot * a.shot_stride + (a.axis == 0 ? (long long)other * a.sy : (long long)other * a.sx);
+    const long long poff = a.axis == 0 ? (long long)other * a.sy : (long long)other * a.sx;  // plane offset in the shot
+    const long long ubase = (long long)shot * a.shot_stride + poff;
     float2* pen = lds + p * G::PS;
     load_twiddles<G>(tws, a.tw);
     for (int jb = 0; jb < a.njobs; ++jb) {
@@ -129,10 +134,22 @@ void k2_deriv_strided(StridedArgs a)
             if (act1 && zok) {
                 float2 d[G::N1];
                 inv_lb<G>(d, pen, jj);
+                if (a.bnf) {
+                    // blocked group: (ip, ip+1) share a 64-cell block (ip even)
+                    float* ob = J.out + (long long)shot * a.bnf * a.shot_stride;
+                    const unsigned ip0 = (unsigned)(poff + jj * astride + zc);
+                    const unsigned bs = (unsigned)(kBlk * a.bnf);
 #pragma unroll
-                for (int j1 = 0; j1 < G::N1; ++j1) {
-                    const long long urow = ubase + (long long)(G::N2 * j1) * astride;
-                    *reinterpret_cast<float2*>(J.out + urow + voff) = d[j1];
+                    for (int j1 = 0; j1 < G::N1; ++j1) {
+                        const unsigned ip = ip0 + (unsigned)(G::N2 * j1 * astride);
+                        *reinterpret_cast<float2*>(ob + ((ip >> 6) * bs + (ip & 63u))) = d[j1];
+                    }
+                } else {
+#pragma unroll
+                    for (int j1 = 0; j1 < G::N1; ++j1) {
+                        const long long urow = ubase + (long long)(G::N2 * j1) * astride;
+                        *reinterpret_cast<float2*>(J.out + urow + voff) = d[j1];
+                    }
                 }
             }
         } else {
@@ -618,9 +635,12 @@ struct StressZArgs {
     const float *pay, *pby, *payh, *pbyh;
     const float *paz, *pbz, *pazh, *pbzh;
     const float* fam[3][5];  // [component][A1 v, v, A3 v, A1 dv, dv]
-    const float *ex[5], *gx[5], *hx[5];
-    const float *ey[5], *gy[5], *hy[5];
-    const float *c11[5], *c33[5], *c13[5], *cs[5], *csb[5];
+    // x- and y-derivatives, blocked groups of 15 per shot:
+    //   dxb: ex 0-4 (dx vx-family), gx 5-9 (dx vz-family), hx 10-14 (dx vy-family)
+    //   dyb: ey 0-4 (dy vy-family), gy 5-9 (dy vz-family), hy 10-14 (dy vx-family)
+    const float *dxb, *dyb;
+    // coefficient groups G1 {P, S, S-bar} and G2 {H, D} (VTI; else null)
+    const float *g1, *g2;
     float* sig[6];  // pxx pyy pzz pxy pxz pyz
     float* phx[5];
     float* phy[5];
@@ -714,30 +734,49 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
     const int it = *a.it;
     const int kind = a.src_kind[shot];
     const long long sip = a.src_ip[shot];
+    // uniform block bases; per-cell offsets below are unsigned 32-bit
+    const long long cbase = r0 * n;          // flat index of the tile's first cell
+    const long long fbase = soff + cbase;    // per-shot natural fields
+    const float* __restrict__ dxs = a.dxb + (long long)shot * kG1 * a.shot_stride;
+    const float* __restrict__ dys = a.dyb + (long long)shot * kG1 * a.shot_stride;
+    const bool vti = a.g2 != nullptr;
+    const long long sxb = (long long)shot * a.slabx_stride, syb = (long long)shot * a.slaby_stride,
+                    szb = (long long)shot * a.slabz_stride;
 #pragma unroll 1
     for (int cell = tid; cell < R * n; cell += kThreads) {
         const int rr = cell / n, iz = cell - rr * n;
         const long long row = r0 + rr;
         if (row >= a.nrows) continue;
         const int ix = (int)(row / a.ny), iy = (int)(row - (long long)ix * a.ny);
-        const int ip = (int)(row * n + iz);
-        const long long g = soff + ip;
+        const unsigned o = (unsigned)cell;  // == rr * n + iz
         auto S = [&](int f) { return st[(f * R + rr) * n + iz]; };
         if (ix < a.nx - 1 && iy < a.ny - 1 && iz < n - 1) {
             // staging fields of the family members j = 0..4 (A1 v, v, A3 v, A1 dv, dv)
             const int fz[5] = {4, 10, 14, 5, 11};   // vz family (D+): normal dz vz
             const int fgx[5] = {0, 6, 12, 1, 7};    // vx family (D-): dz vx for pxz
             const int fhy[5] = {2, 8, 13, 3, 9};    // vy family (D-): dz vy for pyz
+            // blocked addresses of the cell: one pointer per group, the
+            // fields at immediate offsets f*64 floats
+            const unsigned ip = (unsigned)cbase + o;
+            const unsigned q15 = (ip >> 6) * (kBlk * kG1) + (ip & 63u);
+            const float* __restrict__ dxp = dxs + q15;
+            const float* __restrict__ dyp = dys + q15;
+            const float* __restrict__ g1p = a.g1 + q15;
+            const float* __restrict__ g2p = vti ? a.g2 + ((ip >> 6) * (kBlk * kG2) + (ip & 63u)) : nullptr;
             float A[14];
 #pragma unroll
             for (int k = 0; k < 14; ++k) A[k] = 0.f;
 #pragma unroll
             for (int jf = 0; jf < 5; ++jf) {
                 const float ez = S(fz[jf]), gz = S(fgx[jf]), hz = S(fhy[jf]);
-                const float ex = (a.ex[jf] + soff)[ip], gx = (a.gx[jf] + soff)[ip], hx = (a.hx[jf] + soff)[ip];
-                const float ey = (a.ey[jf] + soff)[ip], gy = (a.gy[jf] + soff)[ip], hy = (a.hy[jf] + soff)[ip];
-                const float c11 = a.c11[jf][ip], c33 = a.c33[jf][ip], c13 = a.c13[jf][ip];
-                const float cs = a.cs[jf][ip], csb = a.csb[jf][ip];
+                const float ex = dxp[jf * kBlk], gx = dxp[(5 + jf) * kBlk], hx = dxp[(10 + jf) * kBlk];
+                const float ey = dyp[jf * kBlk], gy = dyp[(5 + jf) * kBlk], hy = dyp[(10 + jf) * kBlk];
+                const float c33 = g1p[jf * kBlk], cs = g1p[(5 + jf) * kBlk], csb = g1p[(10 + jf) * kBlk];
+                float c11 = c33, c13 = c33;
+                if (vti) {
+                    c11 = g2p[jf * kBlk];
+                    c13 = g2p[(5 + jf) * kBlk];
+                }
                 const float c12 = c11 - 2 * cs, c13d = c13 - 2 * cs;
                 A[0] += c11 * ex;   // s11x
                 A[1] += c12 * ex;   // s12x
@@ -754,59 +793,88 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
                 A[12] += csb * hz;  // sb_yz  (dz vy)
                 A[13] += csb * gy;  // sb_zy  (dy vz)
             }
+            // Every load of the cell is issued before its first store: the
+            // stress and C-PML arrays are distinct, but the compiler cannot
+            // know it, so loads placed after a store would each wait for it.
+            float sg[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sg[k] = a.sig[k][fbase + o];
+            const bool xs = ix < pml || ix >= a.nx - pml;
+            const bool ys = iy < pml || iy >= a.ny - pml;
+            const bool zs = iz < pml || iz >= n - pml;
+            unsigned ox = 0, oy = 0, oz = 0;
+            float qx[5], qy[5], qz[4];
+            if (xs) {
+                const int si = ix < pml ? ix : ix - (a.nx - 2 * pml);
+                ox = (unsigned)(((long long)si * a.ny + iy) * n + iz);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) qx[k] = a.phx[k][sxb + ox];
+            }
+            if (ys) {
+                const int si = iy < pml ? iy : iy - (a.ny - 2 * pml);
+                oy = (unsigned)(((long long)ix * 2 * pml + si) * n + iz);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) qy[k] = a.phy[k][syb + oy];
+            }
+            if (zs) {
+                const int si = iz < pml ? iz : iz - (n - 2 * pml);
+                oz = (unsigned)(row * 2 * pml + si);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) qz[k] = a.phz[k][szb + oz];
+            }
             float fxx_x = 0.f, fxy_x = 0.f, fxz_x = 0.f, fzx = 0.f, fyx = 0.f;
             float fyx_y = 0.f, fyy_y = 0.f, fyz_y = 0.f, fxy = 0.f, fzy = 0.f;
             float fzx_z = 0.f, fzz_z = 0.f, fxz = 0.f, fyz = 0.f;
-            if (ix < pml || ix >= a.nx - pml) {
-                const int si = ix < pml ? ix : ix - (a.nx - 2 * pml);
-                const long long s = (long long)shot * a.slabx_stride + ((long long)si * a.ny + iy) * n + iz;
+            if (xs) {
                 const float Ax = a.pax[ix], Bx = a.pbx[ix], Ah = a.paxh[ix], Bh = a.pbxh[ix];
-                float* p;
-                p = a.phx[0] + s; fxx_x = *p + dt * (-Ax * *p - Bx * A[0]); *p = fxx_x;
-                p = a.phx[1] + s; fxy_x = *p + dt * (-Ax * *p - Bx * A[1]); *p = fxy_x;
-                p = a.phx[2] + s; fxz_x = *p + dt * (-Ax * *p - Bx * A[2]); *p = fxz_x;
-                p = a.phx[3] + s; fzx = *p + dt * (-Ah * *p - Bh * A[9]); *p = fzx;
-                p = a.phx[4] + s; fyx = *p + dt * (-Ah * *p - Bh * A[11]); *p = fyx;
+                fxx_x = qx[0] + dt * (-Ax * qx[0] - Bx * A[0]);
+                fxy_x = qx[1] + dt * (-Ax * qx[1] - Bx * A[1]);
+                fxz_x = qx[2] + dt * (-Ax * qx[2] - Bx * A[2]);
+                fzx = qx[3] + dt * (-Ah * qx[3] - Bh * A[9]);
+                fyx = qx[4] + dt * (-Ah * qx[4] - Bh * A[11]);
+                a.phx[0][sxb + ox] = fxx_x;
+                a.phx[1][sxb + ox] = fxy_x;
+                a.phx[2][sxb + ox] = fxz_x;
+                a.phx[3][sxb + ox] = fzx;
+                a.phx[4][sxb + ox] = fyx;
             }
-            if (iy < pml || iy >= a.ny - pml) {
-                const int si = iy < pml ? iy : iy - (a.ny - 2 * pml);
-                const long long s = (long long)shot * a.slaby_stride + ((long long)ix * 2 * pml + si) * n + iz;
+            if (ys) {
                 const float Ay = a.pay[iy], By = a.pby[iy], Ah = a.payh[iy], Bh = a.pbyh[iy];
-                float* p;
-                p = a.phy[0] + s; fyx_y = *p + dt * (-Ay * *p - By * A[3]); *p = fyx_y;
-                p = a.phy[1] + s; fyy_y = *p + dt * (-Ay * *p - By * A[4]); *p = fyy_y;
-                p = a.phy[2] + s; fyz_y = *p + dt * (-Ay * *p - By * A[5]); *p = fyz_y;
-                p = a.phy[3] + s; fxy = *p + dt * (-Ah * *p - Bh * A[10]); *p = fxy;
-                p = a.phy[4] + s; fzy = *p + dt * (-Ah * *p - Bh * A[13]); *p = fzy;
+                fyx_y = qy[0] + dt * (-Ay * qy[0] - By * A[3]);
+                fyy_y = qy[1] + dt * (-Ay * qy[1] - By * A[4]);
+                fyz_y = qy[2] + dt * (-Ay * qy[2] - By * A[5]);
+                fxy = qy[3] + dt * (-Ah * qy[3] - Bh * A[10]);
+                fzy = qy[4] + dt * (-Ah * qy[4] - Bh * A[13]);
+                a.phy[0][syb + oy] = fyx_y;
+                a.phy[1][syb + oy] = fyy_y;
+                a.phy[2][syb + oy] = fyz_y;
+                a.phy[3][syb + oy] = fxy;
+                a.phy[4][syb + oy] = fzy;
             }
-            if (iz < pml || iz >= n - pml) {
-                const int si = iz < pml ? iz : iz - (n - 2 * pml);
-                const long long s = (long long)shot * a.slabz_stride + row * 2 * pml + si;
+            if (zs) {
                 const float Az = a.paz[iz], Bz = a.pbz[iz], Ah = a.pazh[iz], Bh = a.pbzh[iz];
-                float* p;
-                p = a.phz[0] + s; fzx_z = *p + dt * (-Az * *p - Bz * A[6]); *p = fzx_z;
-                p = a.phz[1] + s; fzz_z = *p + dt * (-Az * *p - Bz * A[7]); *p = fzz_z;
-                p = a.phz[2] + s; fxz = *p + dt * (-Ah * *p - Bh * A[8]); *p = fxz;
-                p = a.phz[3] + s; fyz = *p + dt * (-Ah * *p - Bh * A[12]); *p = fyz;
+                fzx_z = qz[0] + dt * (-Az * qz[0] - Bz * A[6]);
+                fzz_z = qz[1] + dt * (-Az * qz[1] - Bz * A[7]);
+                fxz = qz[2] + dt * (-Ah * qz[2] - Bh * A[8]);
+                fyz = qz[3] + dt * (-Ah * qz[3] - Bh * A[12]);
+                a.phz[0][szb + oz] = fzx_z;
+                a.phz[1][szb + oz] = fzz_z;
+                a.phz[2][szb + oz] = fxz;
+                a.phz[3][szb + oz] = fyz;
             }
-            float* pxx = a.sig[0] + g;
-            float* pyy = a.sig[1] + g;
-            float* pzz = a.sig[2] + g;
-            float* pxy = a.sig[3] + g;
-            float* pxz = a.sig[4] + g;
-            float* pyz = a.sig[5] + g;
-            *pxx = *pxx + dt * (A[0] + A[3] + A[6] + (fxx_x + fyx_y + fzx_z));
-            *pyy = *pyy + dt * (A[1] + A[4] + A[6] + (fxy_x + fyy_y + fzx_z));
-            *pzz = *pzz + dt * (A[2] + A[5] + A[7] + (fxz_x + fyz_y + fzz_z));
-            *pxz = *pxz + dt * (A[8] + fxz + A[9] + fzx);
-            *pyz = *pyz + dt * (A[12] + fyz + A[13] + fzy);
-            *pxy = *pxy + dt * (A[10] + fxy + A[11] + fyx);
+            a.sig[0][fbase + o] = sg[0] + dt * (A[0] + A[3] + A[6] + (fxx_x + fyx_y + fzx_z));
+            a.sig[1][fbase + o] = sg[1] + dt * (A[1] + A[4] + A[6] + (fxy_x + fyy_y + fzx_z));
+            a.sig[2][fbase + o] = sg[2] + dt * (A[2] + A[5] + A[7] + (fxz_x + fyz_y + fzz_z));
+            a.sig[3][fbase + o] = sg[3] + dt * (A[10] + fxy + A[11] + fyx);
+            a.sig[4][fbase + o] = sg[4] + dt * (A[8] + fxz + A[9] + fzx);
+            a.sig[5][fbase + o] = sg[5] + dt * (A[12] + fyz + A[13] + fzy);
         }
-        if (ip == sip) {
+        if (row * n + iz == sip) {
             // source after the stress update; this workgroup is the only one
             // that reads this row of v (its z-derivative was staged above),
             // so the write cannot race.
             const float s = a.rik[it];
+            const long long g = fbase + o;
             if (kind == 3) {
                 a.sig[0][g] += s;
                 a.sig[1][g] += s;
