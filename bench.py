@@ -85,7 +85,10 @@ def run_plain(name, steps, warmup):
     t0 = time.time()
     e.set_model_dict(m)
     setup_s = time.time() - t0
+    t0 = time.perf_counter()
     e.begin(nt, rik, [qvti.make_shot(sh["kind"], sh["src"], sh["rx"], sh["ry"], sh["rz"])])
+    begin_s = time.perf_counter() - t0
+    tiles = {"pencils": e.tile_choice(), "split": e.tile_split()}
     e.step(warmup)
     e.sync()
     t0 = time.perf_counter()
@@ -100,7 +103,7 @@ def run_plain(name, steps, warmup):
     wall_timed = time.perf_counter() - t0
     kt = e.kernel_times()
     e.close()
-    return g, wall, wall_timed, kt, setup_s
+    return g, wall, wall_timed, kt, setup_s, begin_s, tiles
 
 
 def measured_traffic(name, kernel):
@@ -347,7 +350,7 @@ def main():
         print(json.dumps(run_served(args.served)), flush=True)
         return
     name = args.workload
-    g, wall, wall_timed, kt, setup_s = run_plain(name, args.steps, args.warmup)
+    g, wall, wall_timed, kt, setup_s, begin_s, tiles = run_plain(name, args.steps, args.warmup)
     N = g["nx"] * g["ny"] * g["nz"]
     gcs = N * args.steps / wall / 1e9
     balg = b_alg(g["nx"], g["ny"], g["nz"], g["pml"], g["vti"])
@@ -387,6 +390,8 @@ def main():
         "kernels": per_kernel,
         "ms_per_step_event_timed": round(wall_timed / args.steps * 1e3, 4),
         "setup_s": round(setup_s, 3),
+        "begin_s": round(begin_s, 3),
+        "tile_pencils": tiles,
     }
     if not args.no_cpu_baseline:
         try:

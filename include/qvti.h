@@ -144,6 +144,19 @@ int qvti_kernel_times(qvti_ctx *c, int max_classes, char *names, int name_len, l
 int qvti_kernel_bytes(qvti_ctx *c, int max_classes, double *bytes);
 /* Use HIP graphs for chunks of steps (default on). */
 int qvti_set_graphs(qvti_ctx *c, int on);
+/* Tile width of the strided-axis kernels.  mode 1 (default): autotune - at
+ * the first qvti_begin for a shot count, every available width of each
+ * strided kernel class is timed on this context and the fastest kept.
+ * mode 0: the narrowest tiles; mode 2 or 4: that width (or the widest
+ * available below it) for every class; + 16: one field per workgroup.
+ * Results are identical for every choice: a pencil's arithmetic does not
+ * depend on the tile. */
+int qvti_set_autotune(qvti_ctx *c, int mode);
+/* Pencils per tile and workgroups per tile (1: all fields of the launch in
+ * one workgroup; else one field per workgroup) chosen for each kernel class
+ * (0 for row kernels), in the class order of qvti_kernel_times; split may be
+ * NULL.  Returns the number of classes. */
+int qvti_tile_choice(qvti_ctx *c, int max_classes, int *pencils, int *split);
 /* Blow-up probe interval in steps (0 disables; default 50) and limit on |v|. */
 int qvti_set_probe(qvti_ctx *c, int every, float limit);
 /* Device bytes this context holds. */

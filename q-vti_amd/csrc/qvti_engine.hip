@@ -73,6 +73,58 @@ static bool dispatch_n(int n, F&& f)
 }
 static const char* kSupportedList = "16 20 24 32 40 48 64 96 100 128 160 200 256 400";
 
+// Tile widths of the strided-axis kernels: PW = 1, 2, 4 (256, 512, 1024
+// threads; P = 256*PW/NMAX pencils).  Wider tiles read longer contiguous
+// z-runs per row; they exist for n >= 64 and while the tile fits in LDS.
+template <int A, int B, class F>
+static void call_pw(int pw, F&& f)
+{
+    if constexpr (A * B >= 64 && L2<A, B, 4>::kFits) {
+        if (pw >= 4) { f(L2<A, B, 4>{}); return; }
+    }
+    if constexpr (A * B >= 64 && L2<A, B, 2>::kFits) {
+        if (pw >= 2) { f(L2<A, B, 2>{}); return; }
+    }
+    f(L2<A, B, 1>{});
+}
+
+// Run f(L2<N1,N2,PW'>{}) for length n with the widest supported PW' <= pw.
+template <class F>
+static bool dispatch_nw(int n, int pw, F&& f)
+{
+    switch (n) {
+    case 16: call_pw<4, 4>(pw, f); return true;
+    case 20: call_pw<4, 5>(pw, f); return true;
+    case 24: call_pw<4, 6>(pw, f); return true;
+    case 32: call_pw<4, 8>(pw, f); return true;
+    case 40: call_pw<5, 8>(pw, f); return true;
+    case 48: call_pw<6, 8>(pw, f); return true;
+    case 64: call_pw<8, 8>(pw, f); return true;
+    case 96: call_pw<8, 12>(pw, f); return true;
+    case 100: call_pw<10, 10>(pw, f); return true;
+    case 128: call_pw<8, 16>(pw, f); return true;
+    case 160: call_pw<10, 16>(pw, f); return true;
+    case 200: call_pw<10, 20>(pw, f); return true;
+    case 256: call_pw<16, 16>(pw, f); return true;
+    case 400: call_pw<20, 20>(pw, f); return true;
+    default: return false;
+    }
+}
+
+// Effective (PW, pencils per tile, threads) of length n at requested width pw.
+struct TileW {
+    int pw, P, T;
+};
+static TileW tile_w(int n, int pw)
+{
+    TileW t{0, 0, 0};
+    dispatch_nw(n, pw, [&](auto g) {
+        using G = decltype(g);
+        t = TileW{G::PW, G::P, G::T};
+    });
+    return t;
+}
+
 static int tile_pencils(int n)
 {
     int P = 0;
@@ -215,8 +267,15 @@ struct qvti_ctx {
     XPassArgs a_xp{};
     StressZArgs a_sz{};
     RecArgs a_rec{};
-    dim3 g_vx, g_vy, g_vz, g_zr, g_cy, g_xp, g_sz, g_rec;
+    dim3 g_vx, g_vy, g_vz, g_zr, g_cy, g_xp, g_sz, g_rec, g_sx, g_sy, g_yi;
     int P[3] = {0, 0, 0};
+    // tile width per kernel class (strided-axis kernels), chosen by the autotuner
+    int pw[KC_COUNT] = {};
+    // fields per workgroup of the strided kernels: grid z = 1 (every field in
+    // one workgroup) or the field count (one field per workgroup)
+    int jz[KC_COUNT] = {};
+    bool autotune = true;
+    int tuned = 0;  // shot count the tile widths were tuned for (0: not tuned)
     long long bytes = 0;
     std::vector<void*> allocs;
 };
@@ -285,6 +344,7 @@ extern "C" long long qvti_device_bytes(qvti_ctx* c) { return c ? c->bytes : -1; 
 hipStream_t qvti_ctx_stream(qvti_ctx* c) { return c ? c->stream : nullptr; }
 
 static int build_launch_config(qvti_ctx* c);
+static void apply_tiles(qvti_ctx* c);
 
 extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, int device)
 {
@@ -430,7 +490,6 @@ static int build_launch_config(qvti_ctx* c)
     vx.job[1] = job(pxy, acc[1], nullptr, nullptr, 0, 0);
     vx.job[2] = job(pxz, acc[2], nullptr, nullptr, 0, 0);
     for (int k = 0; k < 3; ++k) vx.job[k].phi = c->phiv[0] + (long long)k * c->S * c->slab[0];  // [field][shot][slab]
-    c->g_vx = dim3(ny * vx.ztiles, 1, 1);
 
     // ---- velocity y: pxy (D+) -> acc_x, pyy (D-, half) -> acc_y, pyz (D+) -> acc_z
     StridedArgs& vy = c->a_vy;
@@ -441,7 +500,6 @@ static int build_launch_config(qvti_ctx* c)
     vy.job[1] = job(pyy, acc[1], acc[1], nullptr, 1, 1);
     vy.job[2] = job(pyz, acc[2], acc[2], nullptr, 0, 0);
     for (int k = 0; k < 3; ++k) vy.job[k].phi = c->phiv[1] + (long long)k * c->S * c->slab[1];
-    c->g_vy = dim3(nx * vy.ztiles, 1, 1);
 
     // ---- 3-D transform buffers: P_c = FFT(v_c + i dv_c) (bufB as complex),
     // Q1_c, Q3_c (bufA as complex)
@@ -495,7 +553,6 @@ static int build_launch_config(qvti_ctx* c)
     yf.tw = Y.tw;
     yf.nfields = 3;
     for (int k = 0; k < 3; ++k) yf.f[k] = P[k];
-    c->g_cy = dim3(nx * yf.ztiles, 1, 1);
 
     XPassArgs& xp = c->a_xp;
     xp.ny = ny; xp.nz = nz; xp.sx = sx; xp.sy = sy;
@@ -506,7 +563,6 @@ static int build_launch_config(qvti_ctx* c)
     xp.invN = 1.0 / (double)N;
     xp.ncomp = 3;
     for (int k = 0; k < 3; ++k) { xp.P[k] = P[k]; xp.Q1[k] = Q1[k]; xp.Q3[k] = Q3[k]; }
-    c->g_xp = dim3(ny * xp.ztiles, 1, 1);
 
     CStridedArgs& yi = c->a_yi;
     yi = yf;
@@ -592,6 +648,8 @@ static int build_launch_config(qvti_ctx* c)
     rec.rec_off = c->d_rec_off;
     rec.it = c->d_it;
     (void)pyy;
+    for (int k = 0; k < KC_COUNT; ++k) c->pw[k] = c->jz[k] = 1;
+    apply_tiles(c);
     return QVTI_OK;
 }
 
@@ -655,50 +713,170 @@ static int record_event(qvti_ctx* c, int cls)
 
 // Enqueue one time step (14 launches).  With timing on, each launch is
 // bracketed by an event pair.
-static void enqueue_step(qvti_ctx* c)
+// Launch one kernel of class cls on the context's stream.
+static void launch_class(qvti_ctx* c, int cls)
 {
     hipStream_t st = c->stream;
     const dim3 blk(kThreads);
     const unsigned S = (unsigned)c->nshots;
     const int nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
-    auto timed = [&](int cls, auto&& launch) {
-        const int e0 = c->timing ? record_event(c, cls) : -1;
-        launch();
-        if (e0 >= 0) record_event(c, -1 - cls);
-    };
-    timed(KC_TICK, [&] { hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, c->d_it); });
-    timed(KC_VEL_X, [&] {
-        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_VEL_FIRST>), dim3(c->g_vx.x, S), blk, 0, st, c->a_vx); });
-    });
-    timed(KC_VEL_Y, [&] {
-        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_VEL_ACC>), dim3(c->g_vy.x, S), blk, 0, st, c->a_vy); });
-    });
-    timed(KC_VEL_Z, [&] {
+    switch (cls) {
+    case KC_TICK:
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, c->d_it);
+        break;
+    case KC_VEL_X:
+        dispatch_nw(nx, c->pw[cls], [&](auto g) {
+            using G = decltype(g);
+            hipLaunchKernelGGL((k2_deriv_strided<G, MODE_VEL_FIRST>), dim3(c->g_vx.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_vx);
+        });
+        break;
+    case KC_VEL_Y:
+        dispatch_nw(ny, c->pw[cls], [&](auto g) {
+            using G = decltype(g);
+            hipLaunchKernelGGL((k2_deriv_strided<G, MODE_VEL_ACC>), dim3(c->g_vy.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_vy);
+        });
+        break;
+    case KC_VEL_Z:
         dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_vel_z<decltype(g)>, dim3(c->g_vz.x, S), blk, 0, st, c->a_vz); });
-    });
-    timed(KC_Y_FWD, [&] {
-        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL((k2_cfft_strided<decltype(g), -1>), dim3(c->g_cy.x, S), blk, 0, st, c->a_yf); });
-    });
-    timed(KC_XPASS, [&] {
-        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL(k2_xpass<decltype(g)>, dim3(c->g_xp.x, S), blk, 0, st, c->a_xp); });
-    });
-    timed(KC_Y_INV, [&] {
-        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL((k2_cfft_strided<decltype(g), 1>), dim3(c->g_cy.x, S), blk, 0, st, c->a_yi); });
-    });
-    timed(KC_Z_INV, [&] {
+        break;
+    case KC_Y_FWD:
+        dispatch_nw(ny, c->pw[cls], [&](auto g) {
+            using G = decltype(g);
+            hipLaunchKernelGGL((k2_cfft_strided<G, -1>), dim3(c->g_cy.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_yf);
+        });
+        break;
+    case KC_XPASS:
+        dispatch_nw(nx, c->pw[cls], [&](auto g) {
+            using G = decltype(g);
+            hipLaunchKernelGGL(k2_xpass<G>, dim3(c->g_xp.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_xp);
+        });
+        break;
+    case KC_Y_INV:
+        dispatch_nw(ny, c->pw[cls], [&](auto g) {
+            using G = decltype(g);
+            hipLaunchKernelGGL((k2_cfft_strided<G, 1>), dim3(c->g_yi.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_yi);
+        });
+        break;
+    case KC_Z_INV:
         dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3(c->g_zr.x, S), blk, 0, st, c->a_zi); });
-    });
-    timed(KC_STR_X, [&] {
-        dispatch_n(nx, [&](auto g) { hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_STORE>), dim3(c->g_vx.x, S), blk, 0, st, c->a_sx); });
-    });
-    timed(KC_STR_Y, [&] {
-        dispatch_n(ny, [&](auto g) { hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_STORE>), dim3(c->g_vy.x, S), blk, 0, st, c->a_sy); });
-    });
-    timed(KC_STR_Z, [&] {
+        break;
+    case KC_STR_X:
+        dispatch_nw(nx, c->pw[cls], [&](auto g) {
+            using G = decltype(g);
+            hipLaunchKernelGGL((k2_deriv_strided<G, MODE_STORE>), dim3(c->g_sx.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_sx);
+        });
+        break;
+    case KC_STR_Y:
+        dispatch_nw(ny, c->pw[cls], [&](auto g) {
+            using G = decltype(g);
+            hipLaunchKernelGGL((k2_deriv_strided<G, MODE_STORE>), dim3(c->g_sy.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_sy);
+        });
+        break;
+    case KC_STR_Z:
         dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_stress_z<decltype(g)>, dim3(c->g_sz.x, S), blk, 0, st, c->a_sz); });
-    });
-    if (c->total_rec > 0)
-        timed(KC_RECORD, [&] { hipLaunchKernelGGL(k_record, dim3(c->g_rec.x, S), blk, 0, st, c->a_rec); });
+        break;
+    case KC_RECORD:
+        hipLaunchKernelGGL(k_record, dim3(c->g_rec.x, S), blk, 0, st, c->a_rec);
+        break;
+    default:
+        break;
+    }
+}
+
+// Enqueue one time step (12 launches).  With timing on, each launch is
+// bracketed by an event pair.
+static void enqueue_step(qvti_ctx* c)
+{
+    static const int order[] = {KC_TICK, KC_VEL_X, KC_VEL_Y, KC_VEL_Z, KC_Y_FWD, KC_XPASS,
+                                KC_Y_INV, KC_Z_INV, KC_STR_X, KC_STR_Y, KC_STR_Z, KC_RECORD};
+    for (int cls : order) {
+        if (cls == KC_RECORD && c->total_rec == 0) continue;
+        const int e0 = c->timing ? record_event(c, cls) : -1;
+        launch_class(c, cls);
+        if (e0 >= 0) record_event(c, -1 - cls);
+    }
+}
+
+// Geometry of the strided-axis launches for the current tile widths.
+static void apply_tiles(qvti_ctx* c)
+{
+    const int nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
+    auto zt = [&](int n, int cls, int cols) {
+        const int w = tile_w(n, c->pw[cls]).P * cols;  // z-columns per tile
+        return (nz + w - 1) / w;
+    };
+    c->a_vx.ztiles = zt(nx, KC_VEL_X, 2); c->g_vx = dim3(ny * c->a_vx.ztiles);
+    c->a_sx.ztiles = zt(nx, KC_STR_X, 2); c->g_sx = dim3(ny * c->a_sx.ztiles);
+    c->a_vy.ztiles = zt(ny, KC_VEL_Y, 2); c->g_vy = dim3(nx * c->a_vy.ztiles);
+    c->a_sy.ztiles = zt(ny, KC_STR_Y, 2); c->g_sy = dim3(nx * c->a_sy.ztiles);
+    c->a_yf.ztiles = zt(ny, KC_Y_FWD, 1); c->g_cy = dim3(nx * c->a_yf.ztiles);
+    c->a_yi.ztiles = zt(ny, KC_Y_INV, 1); c->g_yi = dim3(nx * c->a_yi.ztiles);
+    c->a_xp.ztiles = zt(nx, KC_XPASS, 1); c->g_xp = dim3(ny * c->a_xp.ztiles);
+}
+
+// Fields (jobs) one launch of a strided kernel class processes.
+static int class_fields(int cls)
+{
+    switch (cls) {
+    case KC_VEL_X: case KC_VEL_Y: case KC_Y_FWD: case KC_XPASS: return 3;
+    case KC_Y_INV: return 6;
+    case KC_STR_X: case KC_STR_Y: return 15;
+    default: return 1;
+    }
+}
+
+// Autotuner: time every available tile width of each strided-axis kernel
+// class on this context's grid and shot count, keep the fastest.  Runs on the
+// context's own buffers before a run starts: the velocity-step kernels see
+// zero stresses (so the C-PML memory variables stay zero) and every other
+// tuned kernel only writes scratch buffers that a step overwrites first.
+static int autotune_tiles(qvti_ctx* c)
+{
+    static const int classes[] = {KC_VEL_X, KC_VEL_Y, KC_Y_FWD, KC_XPASS, KC_Y_INV, KC_STR_X, KC_STR_Y};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return qvti_fail(QVTI_E_DEVICE, "hipEventCreate failed");
+    }
+    int rc = QVTI_OK;
+    for (int cls : classes) {
+        const int n = (cls == KC_VEL_X || cls == KC_XPASS || cls == KC_STR_X) ? c->g.nx : c->g.ny;
+        const int nf = class_fields(cls);
+        int best_pw = 1, best_jz = 1;
+        float best = 1e30f;
+        for (int pw : {1, 2, 4}) {
+            if (tile_w(n, pw).pw != pw) continue;  // width not available for n
+            for (int jz : {1, nf}) {
+                c->pw[cls] = pw;
+                c->jz[cls] = jz;
+                apply_tiles(c);
+                launch_class(c, cls);  // warm-up
+                hipEventRecord(e0, c->stream);
+                for (int r = 0; r < 3; ++r) launch_class(c, cls);
+                hipEventRecord(e1, c->stream);
+                float ms = 0.f;
+                if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    rc = qvti_fail(QVTI_E_DEVICE, "autotune timing failed");
+                    break;
+                }
+                if (ms < best) {
+                    best = ms;
+                    best_pw = pw;
+                    best_jz = jz;
+                }
+            }
+            if (rc) break;
+        }
+        c->pw[cls] = best_pw;
+        c->jz[cls] = best_jz;
+        if (rc) break;
+    }
+    apply_tiles(c);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (!rc && hipGetLastError() != hipSuccess) rc = qvti_fail(QVTI_E_DEVICE, "autotune launch failed");
+    return rc;
 }
 
 static const float* snap_source(qvti_ctx* c)
@@ -835,6 +1013,11 @@ extern "C" int qvti_begin(qvti_ctx* c, int nt, const float* rik, const qvti_shot
     int maxr = 1;
     for (int s = 0; s < nshots; ++s) maxr = std::max(maxr, shots[s].nrec);
     c->g_rec = dim3((unsigned)((maxr + kThreads - 1) / kThreads), 1, 1);
+    if (c->autotune && c->tuned != nshots) {
+        rc = autotune_tiles(c);
+        if (rc) return rc;
+        c->tuned = nshots;
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->begun = 1;
     if (c->use_graphs) {
@@ -1013,6 +1196,39 @@ extern "C" int qvti_set_graphs(qvti_ctx* c, int on)
     if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
     c->use_graphs = on != 0;
     return QVTI_OK;
+}
+
+extern "C" int qvti_set_autotune(qvti_ctx* c, int mode)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    const bool split = (mode & 16) != 0;
+    const int m = mode & ~16;
+    if ((m != 0 && m != 1 && m != 2 && m != 4) || (m == 1 && split))
+        return qvti_fail(QVTI_E_ARG, "autotune mode %d not in {0, 1, 2, 4, 16, 18, 20}", mode);
+    c->autotune = m == 1;
+    c->tuned = 0;
+    const int w = m == 0 ? 1 : m;
+    if (!c->autotune)
+        for (int k = 0; k < KC_COUNT; ++k) {
+            c->pw[k] = w;  // the widest available <= w is used
+            c->jz[k] = split ? class_fields(k) : 1;
+        }
+    apply_tiles(c);
+    return QVTI_OK;
+}
+
+extern "C" int qvti_tile_choice(qvti_ctx* c, int max_classes, int* pencils, int* split)
+{
+    if (!c || !pencils) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    const int n = std::min(max_classes, (int)KC_COUNT);
+    for (int k = 0; k < n; ++k) {
+        int len = 0;
+        if (k == KC_VEL_X || k == KC_XPASS || k == KC_STR_X) len = c->g.nx;
+        if (k == KC_VEL_Y || k == KC_Y_FWD || k == KC_Y_INV || k == KC_STR_Y) len = c->g.ny;
+        pencils[k] = len ? tile_w(len, c->pw[k]).P : 0;
+        if (split) split[k] = len ? c->jz[k] : 0;
+    }
+    return n;
 }
 
 extern "C" int qvti_set_probe(qvti_ctx* c, int every, float limit)

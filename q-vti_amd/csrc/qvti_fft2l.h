@@ -231,12 +231,17 @@ constexpr int split_n1(int n)
     }
 }
 
-// Per-length constants of the two-level scheme.
-template <int N1_, int N2_>
+// Per-length constants of the two-level scheme.  PW widens the tile of the
+// strided-axis kernels: a workgroup of T = 256*PW threads holds P = T/NMAX
+// pencils, so each row of the tile is P (complex) or 2P (real) contiguous
+// z-columns wide.
+template <int N1_, int N2_, int PW_ = 1>
 struct L2 {
     static constexpr int N1 = N1_, N2 = N2_, N = N1_ * N2_;
     static constexpr int NMAX = N1 > N2 ? N1 : N2;
-    static constexpr int P = 256 / NMAX;          // pencils per 256-thread tile
+    static constexpr int PW = PW_;
+    static constexpr int T = 256 * PW_;           // threads per workgroup
+    static constexpr int P = T / NMAX;            // pencils per tile
     // LDS layout of one pencil: Y rows k1 of length N2 padded by one, Z rows ja
     // of length N1 padded by one; the pencil stride is odd (bank spread).
     static constexpr int YS = N2 + 1, ZS = N1 + 1;
@@ -248,6 +253,8 @@ struct L2 {
     // offsets.  For N1 == N2 the two tables coincide.
     static constexpr int T2OFF = N1 == N2 ? 0 : N;
     static constexpr int TWN = N + T2OFF;
+    // LDS of a strided tile (pencils + twiddles) within the 160 KiB of a CU
+    static constexpr bool kFits = 8L * (P * PS + TWN) <= 160L * 1024;
 };
 
 // Fill the twiddle tables (G::TWN entries) from tw[m] = W_N^m.

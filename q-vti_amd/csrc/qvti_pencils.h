@@ -44,7 +44,7 @@ enum { MODE_STORE = 0, MODE_VEL_FIRST = 1, MODE_VEL_ACC = 2 };
 template <class G>
 __device__ __forceinline__ void load_twiddles(float2* tws, const float2* tw)
 {
-    for (int t = threadIdx.x; t < G::TWN; t += kThreads) twiddle_entry<G>(tws, tw, t);
+    for (int t = threadIdx.x; t < G::TWN; t += G::T) twiddle_entry<G>(tws, tw, t);
 }
 
 // ------------------------------------------------------------------------
@@ -83,7 +83,7 @@ struct StridedArgs {
 };
 
 template <class G, int MODE>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_STRIDED_WAVES)))
+__global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_STRIDED_WAVES)))
 void k2_deriv_strided(StridedArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
@@ -104,7 +104,7 @@ void k2_deriv_strided(StridedArgs a)
     const long long ubase = (long long)shot * a.shot_stride + poff;
     float2* pen = lds + p * G::PS;
     load_twiddles<G>(tws, a.tw);
-    for (int jb = 0; jb < a.njobs; ++jb) {
+    for (int jb = blockIdx.z; jb < a.njobs; jb += gridDim.z) {
         const DJob J = a.job[jb];
         __syncthreads();
         if (act1) {
@@ -345,7 +345,7 @@ struct CStridedArgs {
 };
 
 template <class G, int DIR>
-__global__ __launch_bounds__(kThreads) void k2_cfft_strided(CStridedArgs a)
+__global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(kThreads) void k2_cfft_strided(CStridedArgs a)
     const long long soff = (long long)shot * a.shot_stride;
     float2* pen = lds + p * G::PS;
     load_twiddles<G>(tws, a.tw);
-    for (int f = 0; f < a.nfields; ++f) {
+    for (int f = blockIdx.z; f < a.nfields; f += gridDim.z) {
         float2* base = a.f[f] + soff + obase;
         __syncthreads();
         if (DIR < 0) {
@@ -416,7 +416,7 @@ struct XPassArgs {
 };
 
 template <class G>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS_WAVES))) void k2_xpass(XPassArgs a)
+__global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS_WAVES))) void k2_xpass(XPassArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_X
     const double ky = a.kfy[iy];
     const double kz = a.kfz[zok ? zc : 0];
     const double kyz = ky * ky + kz * kz;
-    for (int comp = 0; comp < a.ncomp; ++comp) {
+    for (int comp = blockIdx.z; comp < a.ncomp; comp += gridDim.z) {
         const float2* src = a.P[comp] + soff + obase;
         __syncthreads();
         if (act1) {

@@ -26,7 +26,7 @@ EXPORTS = [
     "qvti_set_probe", "qvti_device_bytes", "qvti_estimate_bytes", "qvti_fft3d", "qvti_deriv1d",
     "qvti_ricker", "qvti_gamma_from_q", "qvti_reference_receivers", "qvti_make_model",
     "qvti_server_create", "qvti_server_submit", "qvti_server_poll", "qvti_server_wait",
-    "qvti_server_cancel", "qvti_server_clock", "qvti_server_destroy",
+    "qvti_server_cancel", "qvti_server_clock", "qvti_server_destroy", "qvti_set_autotune", "qvti_tile_choice",
 ]
 
 
@@ -95,7 +95,7 @@ def lib():
             "qvti_server_create": [vp, i, ll, i, i], "qvti_server_submit": [vp, vp, vp],
             "qvti_server_poll": [vp, ctypes.c_uint64, vp, vp, vp, vp],
             "qvti_server_wait": [vp, ctypes.c_uint64, d], "qvti_server_cancel": [vp, ctypes.c_uint64],
-            "qvti_server_destroy": [vp],
+            "qvti_server_destroy": [vp], "qvti_set_autotune": [vp, i], "qvti_tile_choice": [vp, i, vp, vp],
         }
         for name, args in sig.items():
             if not hasattr(L, name):
@@ -247,6 +247,25 @@ class Engine:
 
     def set_graphs(self, on):
         check(lib().qvti_set_graphs(self._c, int(on)))
+
+    def set_autotune(self, on):
+        check(lib().qvti_set_autotune(self._c, int(on)))
+
+    def tile_choice(self):
+        """Pencils per tile of each strided kernel class (after the first begin)."""
+        names = ["tick", "vel_x", "vel_y", "vel_z_fwd", "y_fwd", "xpass", "y_inv", "z_inv", "str_x", "str_y"]
+        out = np.zeros(32, np.int32)
+        spl = np.zeros(32, np.int32)
+        k = check(lib().qvti_tile_choice(self._c, 32, _p(out), _p(spl)))
+        return {n: int(out[j]) for j, n in enumerate(names[:k]) if out[j] > 0}
+
+    def tile_split(self):
+        """Workgroups per tile (1 or the field count) of each strided kernel class."""
+        names = ["tick", "vel_x", "vel_y", "vel_z_fwd", "y_fwd", "xpass", "y_inv", "z_inv", "str_x", "str_y"]
+        out = np.zeros(32, np.int32)
+        spl = np.zeros(32, np.int32)
+        k = check(lib().qvti_tile_choice(self._c, 32, _p(out), _p(spl)))
+        return {n: int(spl[j]) for j, n in enumerate(names[:k]) if out[j] > 0}
 
     def set_probe(self, every, limit=1e30):
         check(lib().qvti_set_probe(self._c, every, limit))

@@ -123,3 +123,30 @@ def test_batched_shots_equal_single_shots_bitwise():
         for x, b in zip(xs, batch):
             single = e.forward(shot["nt"], shot["rik"], shot_of(shot, sx=x))
             assert np.array_equal(single, b)
+
+
+@pytest.mark.parametrize("mode", [0, 2, 4, 16, 20, 1])
+def test_tile_widths_agree_bitwise_and_match_the_oracle(mode):
+    """Every tile choice of the strided-axis kernels (narrow, 2x, 4x wide, one
+    field per workgroup, autotuned) gives bit-identical results; the narrow
+    run is checked against the oracle."""
+    shot, model = small_case(64, 96, 40, 6, 12, O.SRC_EXPLOSIVE, True)
+    with engine_for(shot, model) as e:
+        e.set_autotune(mode)
+        e.begin(shot["nt"], shot["rik"], [shot_of(shot)])
+        tiles = e.tile_choice()
+        e.step(shot["nt"])
+        rec = e.finish()[0]
+        st = e.state()
+    with engine_for(shot, model) as e:
+        e.set_autotune(0)
+        e.begin(shot["nt"], shot["rik"], [shot_of(shot)])
+        tiles0 = e.tile_choice()
+        e.step(shot["nt"])
+        ref = e.finish()[0]
+        if mode == 0:
+            ref64 = O.run(shot, model, "f64", state=True)
+            assert rel_l2(st, ref64["state"]) < TOL
+    if mode & 6:
+        assert all(tiles[k] > tiles0[k] for k in tiles0), (tiles, tiles0)  # wider tiles were used
+    assert np.array_equal(rec, ref), (mode, tiles)
