@@ -562,12 +562,16 @@ static int build_launch_config(qvti_ctx* c)
     xp.kfx = X.kf; xp.kfy = Y.kf; xp.kfz = Z.kf;
     xp.invN = 1.0 / (double)N;
     xp.ncomp = 3;
-    for (int k = 0; k < 3; ++k) { xp.P[k] = P[k]; xp.Q1[k] = Q1[k]; xp.Q3[k] = Q3[k]; }
+    // P0 = F(vx + i vy), P1 = F(vz + i dvx), P2 = F(dvy + i dvz): A1 on all
+    // three, |k| on P0 and P1 (A3 vx + i A3 vy, A3 vz + i unused)
+    for (int k = 0; k < 3; ++k) { xp.P[k] = P[k]; xp.Q1[k] = Q1[k]; xp.Q3[k] = k < 2 ? Q3[k] : nullptr; }
 
     CStridedArgs& yi = c->a_yi;
     yi = yf;
-    yi.nfields = 6;
-    for (int k = 0; k < 3; ++k) { yi.f[k] = Q1[k]; yi.f[3 + k] = Q3[k]; }
+    yi.nfields = 5;
+    for (int k = 0; k < 3; ++k) yi.f[k] = Q1[k];
+    yi.f[3] = Q3[0];
+    yi.f[4] = Q3[1];
 
     ZRowsArgs& zi = c->a_zi;
     zi.nrows = c->nrows;
@@ -575,11 +579,12 @@ static int build_launch_config(qvti_ctx* c)
     zi.tw = Z.tw;
     zi.mode = 1;
     zi.dir = 1;
-    zi.nfields = 6;
-    for (int k = 0; k < 3; ++k) {
-        zi.cin[k] = Q1[k]; zi.re_out[k] = H1v[k]; zi.im_out[k] = H1dv[k];
-        zi.cin[3 + k] = Q3[k]; zi.re_out[3 + k] = H3v[k]; zi.im_out[3 + k] = nullptr;
-    }
+    zi.nfields = 5;
+    zi.cin[0] = Q1[0]; zi.re_out[0] = H1v[0]; zi.im_out[0] = H1v[1];    // A1 vx, A1 vy
+    zi.cin[1] = Q1[1]; zi.re_out[1] = H1v[2]; zi.im_out[1] = H1dv[0];   // A1 vz, A1 dvx
+    zi.cin[2] = Q1[2]; zi.re_out[2] = H1dv[1]; zi.im_out[2] = H1dv[2];  // A1 dvy, A1 dvz
+    zi.cin[3] = Q3[0]; zi.re_out[3] = H3v[0]; zi.im_out[3] = H3v[1];    // A3 vx, A3 vy
+    zi.cin[4] = Q3[1]; zi.re_out[4] = H3v[2]; zi.im_out[4] = nullptr;   // A3 vz
 
     // ---- stress x / y derivatives of the 15 family fields
     const float* fam[3][5];
@@ -819,7 +824,7 @@ static int class_fields(int cls)
 {
     switch (cls) {
     case KC_VEL_X: case KC_VEL_Y: case KC_Y_FWD: case KC_XPASS: return 3;
-    case KC_Y_INV: return 6;
+    case KC_Y_INV: return 5;
     case KC_STR_X: case KC_STR_Y: return 15;
     default: return 1;
     }
@@ -1288,9 +1293,9 @@ extern "C" int qvti_kernel_bytes(qvti_ctx* c, int max_classes, double* bytes)
     b[KC_VEL_Y] = 9 * f + 3 * 2 * sy;
     b[KC_VEL_Z] = (3 + 3 + 3 + 3 + 3 + 6) * f + 3.0 * 4 * c->N + 3 * 2 * sz;  // sig, acc, v r/w, dv, P, buoy
     b[KC_Y_FWD] = 12 * f;
-    b[KC_XPASS] = 6 * f + 12 * f;
-    b[KC_Y_INV] = 24 * f;
-    b[KC_Z_INV] = 12 * f + 9 * f;
+    b[KC_XPASS] = 6 * f + 10 * f;
+    b[KC_Y_INV] = 20 * f;
+    b[KC_Z_INV] = 10 * f + 9 * f;
     b[KC_STR_X] = 30 * f;
     b[KC_STR_Y] = 30 * f;
     b[KC_STR_Z] = (15 + 30 + 12) * f + ncoef * 4.0 * c->N + 2 * (5 * sx + 5 * sy + 4 * sz);

@@ -196,7 +196,7 @@ void k2_deriv_strided(StridedArgs a)
 // Velocity update (cuviscoelas_3D.cu:1018-1101) fused with the z-derivatives
 // and with the first (z) pass of the stress step's 3-D transform:
 //   v += b * (acc_xy + dz sigma + phi_z);  dv = v_new - v_old + src_prev;
-//   P_c = FFT_z(v_c + i dv_c)
+//   P0 = FFT_z(vx + i vy), P1 = FFT_z(vz + i dvx), P2 = FFT_z(dvy + i dvz)
 // dv is the change since the previous step's pre-injection velocity (the
 // reference's V - V_prev, cuviscoelas_3D.cu:517-522, 1124-1131, without 1/dt).
 // Tile: G::P rows (z-pencils).
@@ -276,10 +276,9 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
     const bool rowdom = ok1 && ix < a.nx - 1 && iy < a.ny - 1;
     const long long slab_row = (long long)shot * a.slab_stride + r1 * 2 * a.pml;
 
-    // velocity update of component c from its z-derivative (part of d), then
-    // the forward z-FFT of v + i dv into P[c]
+    // velocity update of component c from its z-derivative (part of d):
+    // writes v and dv
     auto update = [&](int c, const float2 (&d)[G::N1], bool imag) {
-        float2 x[G::N1];
         float* __restrict__ vc = a.v[c] + ubase;
         const float* __restrict__ ac = a.acc[c] + ubase;
         const float* __restrict__ bu = a.buoy[c] + (ubase - (long long)shot * a.shot_stride);
@@ -307,7 +306,18 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
             float dv = vnew - vold;
             if (r1 * n + iz == sip && kind == c) dv += sprev;
             if (ok1) a.dv[c][ubase + o] = dv;
-            x[j1] = make_float2(vnew, dv);
+        }
+    };
+    // forward z-FFT of re + i im (first pass of the 3-D transform) into dst.
+    // The inputs are fields this thread has just written (same elements), so
+    // they come back from cache; holding them in registers instead would cost
+    // occupancy.
+    auto fft_store = [&](const float* ref, const float* imf, float2* dstp) {
+        float2 x[G::N1];
+#pragma unroll
+        for (int j1 = 0; j1 < G::N1; ++j1) {
+            const unsigned o = lrow1 + j + G::N2 * j1;
+            x[j1] = make_float2(ref[ubase + o], imf[ubase + o]);
         }
         __syncthreads();
         if (act1) fwd_l1<G>(x, pen1, tws, j);
@@ -315,18 +325,24 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
         if (ok2) {
             float2 b[G::N2];
             fwd_l2<G>(b, pen2, k1);
-            float2* dst = a.P[c] + ubase + (unsigned)(p2 * n);
+            float2* dst = dstp + ubase + (unsigned)(p2 * n);
 #pragma unroll
             for (int k2 = 0; k2 < G::N2; ++k2) dst[k1 + G::N1 * k2] = b[k2];
         }
     };
 
+    // Packing of the stress step's transforms: P0 = F(vx + i vy),
+    // P1 = F(vz + i dvx), P2 = F(dvy + i dvz).  The |k| operator is then
+    // needed on P0 and P1 only (A3 acts on v, not on dv).
     float2 d[G::N1];
     deriv(a.pxz, a.pyz, a.symp, d);  // dz pxz (vx) + i dz pyz (vy), D+
     update(0, d, false);
     update(1, d, true);
+    fft_store(a.v[0], a.v[1], a.P[0]);
     deriv(a.pzz, nullptr, a.symm, d);  // dz pzz (vz), D-
     update(2, d, false);
+    fft_store(a.v[2], a.dv[0], a.P[1]);
+    fft_store(a.dv[1], a.dv[2], a.P[2]);
 }
 
 // ------------------------------------------------------------------------
@@ -445,8 +461,10 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
         __syncthreads();
         float2 b[G::N2];
         if (act2) fwd_l2<G>(b, pen, jj);
+        const int nops = a.Q3[comp] ? 2 : 1;  // |k| (op 1) only where its output is used
 #pragma unroll
         for (int op = 0; op < 2; ++op) {
+            if (op >= nops) break;
             __syncthreads();
             if (act2) {
                 float2 c[G::N2];
