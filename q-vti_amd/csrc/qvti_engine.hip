@@ -562,9 +562,12 @@ static int build_launch_config(qvti_ctx* c)
     xp.kfx = X.kf; xp.kfy = Y.kf; xp.kfz = Z.kf;
     xp.invN = 1.0 / (double)N;
     xp.ncomp = 3;
-    // P0 = F(vx + i vy), P1 = F(vz + i dvx), P2 = F(dvy + i dvz): A1 on all
-    // three, |k| on P0 and P1 (A3 vx + i A3 vy, A3 vz + i unused)
-    for (int k = 0; k < 3; ++k) { xp.P[k] = P[k]; xp.Q1[k] = Q1[k]; xp.Q3[k] = k < 2 ? Q3[k] : nullptr; }
+    // P0 = F(vx + i vy), P1 = F(dvx + i dvy), P2 = F(vz + i dvz): A1 on all
+    // three, |k| on P0 and P2 (A3 vx + i A3 vy, A3 vz + i unused)
+    for (int k = 0; k < 3; ++k) { xp.P[k] = P[k]; xp.Q1[k] = Q1[k]; }
+    xp.Q3[0] = Q3[0];
+    xp.Q3[1] = nullptr;
+    xp.Q3[2] = Q3[1];
 
     CStridedArgs& yi = c->a_yi;
     yi = yf;
@@ -581,8 +584,8 @@ static int build_launch_config(qvti_ctx* c)
     zi.dir = 1;
     zi.nfields = 5;
     zi.cin[0] = Q1[0]; zi.re_out[0] = H1v[0]; zi.im_out[0] = H1v[1];    // A1 vx, A1 vy
-    zi.cin[1] = Q1[1]; zi.re_out[1] = H1v[2]; zi.im_out[1] = H1dv[0];   // A1 vz, A1 dvx
-    zi.cin[2] = Q1[2]; zi.re_out[2] = H1dv[1]; zi.im_out[2] = H1dv[2];  // A1 dvy, A1 dvz
+    zi.cin[1] = Q1[1]; zi.re_out[1] = H1dv[0]; zi.im_out[1] = H1dv[1];  // A1 dvx, A1 dvy
+    zi.cin[2] = Q1[2]; zi.re_out[2] = H1v[2]; zi.im_out[2] = H1dv[2];   // A1 vz, A1 dvz
     zi.cin[3] = Q3[0]; zi.re_out[3] = H3v[0]; zi.im_out[3] = H3v[1];    // A3 vx, A3 vy
     zi.cin[4] = Q3[1]; zi.re_out[4] = H3v[2]; zi.im_out[4] = nullptr;   // A3 vz
 

@@ -196,7 +196,7 @@ void k2_deriv_strided(StridedArgs a)
 // Velocity update (cuviscoelas_3D.cu:1018-1101) fused with the z-derivatives
 // and with the first (z) pass of the stress step's 3-D transform:
 //   v += b * (acc_xy + dz sigma + phi_z);  dv = v_new - v_old + src_prev;
-//   P0 = FFT_z(vx + i vy), P1 = FFT_z(vz + i dvx), P2 = FFT_z(dvy + i dvz)
+//   P0 = FFT_z(vx + i vy), P1 = FFT_z(dvx + i dvy), P2 = FFT_z(vz + i dvz)
 // dv is the change since the previous step's pre-injection velocity (the
 // reference's V - V_prev, cuviscoelas_3D.cu:517-522, 1124-1131, without 1/dt).
 // Tile: G::P rows (z-pencils).
@@ -276,49 +276,35 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
     const bool rowdom = ok1 && ix < a.nx - 1 && iy < a.ny - 1;
     const long long slab_row = (long long)shot * a.slab_stride + r1 * 2 * a.pml;
 
-    // velocity update of component c from its z-derivative (part of d):
-    // writes v and dv
-    auto update = [&](int c, const float2 (&d)[G::N1], bool imag) {
-        float* __restrict__ vc = a.v[c] + ubase;
-        const float* __restrict__ ac = a.acc[c] + ubase;
-        const float* __restrict__ bu = a.buoy[c] + (ubase - (long long)shot * a.shot_stride);
-#pragma unroll
-        for (int j1 = 0; j1 < G::N1; ++j1) {
-            const int iz = j + G::N2 * j1;
-            const unsigned o = lrow1 + iz;
-            float dz = imag ? d[j1].y : d[j1].x;
-            const float vold = vc[o];
-            float vnew = vold;
-            if (rowdom && iz < n - 1) {
-                if (iz < a.pml || iz >= n - a.pml) {
-                    const int si = iz < a.pml ? iz : iz - (n - 2 * a.pml);
-                    float* ph = a.phi[c] + slab_row + si;
-                    const bool h = c == 2;
-                    const float A = h ? a.pah[iz] : a.pa[iz];
-                    const float B = h ? a.pbh[iz] : a.pb[iz];
-                    const float q = *ph + a.dt * (-A * *ph - B * dz);
-                    *ph = q;
-                    dz = dz + q;
-                }
-                vnew = vold + bu[o] * (ac[o] + dz);
-                if (ok1) vc[o] = vnew;
+    // velocity update of component c at element j1 from its z-derivative dz:
+    // writes v and dv, returns (v_new, dv)
+    auto update1 = [&](int c, int j1, float dz) {
+        const int iz = j + G::N2 * j1;
+        const unsigned o = lrow1 + iz;
+        const float vold = a.v[c][ubase + o];
+        float vnew = vold;
+        if (rowdom && iz < n - 1) {
+            if (iz < a.pml || iz >= n - a.pml) {
+                const int si = iz < a.pml ? iz : iz - (n - 2 * a.pml);
+                float* ph = a.phi[c] + slab_row + si;
+                const bool h = c == 2;
+                const float A = h ? a.pah[iz] : a.pa[iz];
+                const float B = h ? a.pbh[iz] : a.pb[iz];
+                const float q = *ph + a.dt * (-A * *ph - B * dz);
+                *ph = q;
+                dz = dz + q;
             }
-            float dv = vnew - vold;
-            if (r1 * n + iz == sip && kind == c) dv += sprev;
-            if (ok1) a.dv[c][ubase + o] = dv;
+            const float* bu = a.buoy[c] + (ubase - (long long)shot * a.shot_stride);
+            vnew = vold + bu[o] * (a.acc[c][ubase + o] + dz);
+            if (ok1) a.v[c][ubase + o] = vnew;
         }
+        float dv = vnew - vold;
+        if (r1 * n + iz == sip && kind == c) dv += sprev;
+        if (ok1) a.dv[c][ubase + o] = dv;
+        return make_float2(vnew, dv);
     };
-    // forward z-FFT of re + i im (first pass of the 3-D transform) into dst.
-    // The inputs are fields this thread has just written (same elements), so
-    // they come back from cache; holding them in registers instead would cost
-    // occupancy.
-    auto fft_store = [&](const float* ref, const float* imf, float2* dstp) {
-        float2 x[G::N1];
-#pragma unroll
-        for (int j1 = 0; j1 < G::N1; ++j1) {
-            const unsigned o = lrow1 + j + G::N2 * j1;
-            x[j1] = make_float2(ref[ubase + o], imf[ubase + o]);
-        }
+    // forward z-FFT of x (first pass of the 3-D transform) into dst
+    auto fft_store = [&](float2 (&x)[G::N1], float2* dstp) {
         __syncthreads();
         if (act1) fwd_l1<G>(x, pen1, tws, j);
         __syncthreads();
@@ -332,17 +318,30 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
     };
 
     // Packing of the stress step's transforms: P0 = F(vx + i vy),
-    // P1 = F(vz + i dvx), P2 = F(dvy + i dvz).  The |k| operator is then
-    // needed on P0 and P1 only (A3 acts on v, not on dv).
-    float2 d[G::N1];
-    deriv(a.pxz, a.pyz, a.symp, d);  // dz pxz (vx) + i dz pyz (vy), D+
-    update(0, d, false);
-    update(1, d, true);
-    fft_store(a.v[0], a.v[1], a.P[0]);
-    deriv(a.pzz, nullptr, a.symm, d);  // dz pzz (vz), D-
-    update(2, d, false);
-    fft_store(a.v[2], a.dv[0], a.P[1]);
-    fft_store(a.dv[1], a.dv[2], a.P[2]);
+    // P1 = F(dvx + i dvy), P2 = F(vz + i dvz).  The |k| operator is then
+    // needed on P0 and P2 only (A3 acts on v, not on dv).
+    {
+        float2 d[G::N1];
+        deriv(a.pxz, a.pyz, a.symp, d);  // dz pxz (vx) + i dz pyz (vy), D+
+        float2 x0[G::N1], x1[G::N1];
+#pragma unroll
+        for (int j1 = 0; j1 < G::N1; ++j1) {
+            const float2 ux = update1(0, j1, d[j1].x);
+            const float2 uy = update1(1, j1, d[j1].y);
+            x0[j1] = make_float2(ux.x, uy.x);
+            x1[j1] = make_float2(ux.y, uy.y);
+        }
+        fft_store(x0, a.P[0]);
+        fft_store(x1, a.P[1]);
+    }
+    {
+        float2 d[G::N1];
+        deriv(a.pzz, nullptr, a.symm, d);  // dz pzz (vz), D-
+        float2 x2[G::N1];
+#pragma unroll
+        for (int j1 = 0; j1 < G::N1; ++j1) x2[j1] = update1(2, j1, d[j1].x);
+        fft_store(x2, a.P[2]);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -371,19 +370,22 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
     const int zc = (blockIdx.x - other * a.ztiles) * G::P + p;
     const bool zok = zc < a.nz;
     const bool act1 = jj < G::N2 && zok, act2 = jj < G::N1 && zok;
-    const long long astride = a.axis == 0 ? a.sx : a.sy;
-    const long long obase = (a.axis == 0 ? (long long)other * a.sy : (long long)other * a.sx) + zc;
-    const long long soff = (long long)shot * a.shot_stride;
+    const int astride = (int)(a.axis == 0 ? a.sx : a.sy);
+    // uniform tile base (shot, plane, first column) + unsigned lane offset
+    const long long tbase = (long long)shot * a.shot_stride +
+                            (a.axis == 0 ? (long long)other * a.sy : (long long)other * a.sx) +
+                            (long long)(blockIdx.x - other * a.ztiles) * G::P;
+    const unsigned voff = (unsigned)(jj * astride + p);
     float2* pen = lds + p * G::PS;
     load_twiddles<G>(tws, a.tw);
     for (int f = blockIdx.z; f < a.nfields; f += gridDim.z) {
-        float2* base = a.f[f] + soff + obase;
+        float2* const fb = a.f[f] + tbase;
         __syncthreads();
         if (DIR < 0) {
             if (act1) {
                 float2 v[G::N1];
 #pragma unroll
-                for (int j1 = 0; j1 < G::N1; ++j1) v[j1] = base[(long long)(j1 * G::N2 + jj) * astride];
+                for (int j1 = 0; j1 < G::N1; ++j1) v[j1] = (fb + (long long)(j1 * G::N2) * astride)[voff];
                 fwd_l1<G>(v, pen, tws, jj);
             }
             __syncthreads();
@@ -391,13 +393,13 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
                 float2 b[G::N2];
                 fwd_l2<G>(b, pen, jj);
 #pragma unroll
-                for (int k2 = 0; k2 < G::N2; ++k2) base[(long long)(jj + G::N1 * k2) * astride] = b[k2];
+                for (int k2 = 0; k2 < G::N2; ++k2) (fb + (long long)(G::N1 * k2) * astride)[voff] = b[k2];
             }
         } else {
             if (act2) {
                 float2 b[G::N2];
 #pragma unroll
-                for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = base[(long long)(jj + G::N1 * k2) * astride];
+                for (int k2 = 0; k2 < G::N2; ++k2) b[k2] = (fb + (long long)(G::N1 * k2) * astride)[voff];
                 inv_la<G>(b, pen, tws + G::T2OFF, jj);
             }
             __syncthreads();
@@ -405,7 +407,7 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
                 float2 d[G::N1];
                 inv_lb<G>(d, pen, jj);
 #pragma unroll
-                for (int j1 = 0; j1 < G::N1; ++j1) base[(long long)(jj + G::N2 * j1) * astride] = d[j1];
+                for (int j1 = 0; j1 < G::N1; ++j1) (fb + (long long)(G::N2 * j1) * astride)[voff] = d[j1];
             }
         }
     }
@@ -442,20 +444,23 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
     const int zc = (blockIdx.x - iy * a.ztiles) * G::P + p;
     const bool zok = zc < a.nz;
     const bool act1 = jj < G::N2 && zok, act2 = jj < G::N1 && zok;
-    const long long obase = (long long)iy * a.sy + zc;
-    const long long soff = (long long)shot * a.shot_stride;
+    // uniform tile base (shot, plane, first column) + unsigned lane offset
+    const long long tbase = (long long)shot * a.shot_stride + (long long)iy * a.sy +
+                            (long long)(blockIdx.x - iy * a.ztiles) * G::P;
+    const int sxi = (int)a.sx;
+    const unsigned voff = (unsigned)(jj * sxi + p);
     float2* pen = lds + p * G::PS;
     load_twiddles<G>(tws, a.tw);
     const double ky = a.kfy[iy];
     const double kz = a.kfz[zok ? zc : 0];
     const double kyz = ky * ky + kz * kz;
     for (int comp = blockIdx.z; comp < a.ncomp; comp += gridDim.z) {
-        const float2* src = a.P[comp] + soff + obase;
+        const float2* const src = a.P[comp] + tbase;
         __syncthreads();
         if (act1) {
             float2 v[G::N1];
 #pragma unroll
-            for (int j1 = 0; j1 < G::N1; ++j1) v[j1] = src[(long long)(j1 * G::N2 + jj) * a.sx];
+            for (int j1 = 0; j1 < G::N1; ++j1) v[j1] = (src + (long long)(j1 * G::N2) * sxi)[voff];
             fwd_l1<G>(v, pen, tws, jj);
         }
         __syncthreads();
@@ -492,9 +497,9 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
             if (act1) {
                 float2 d[G::N1];
                 inv_lb<G>(d, pen, jj);
-                float2* dst = (op == 0 ? a.Q1[comp] : a.Q3[comp]) + soff + obase;
+                float2* const dst = (op == 0 ? a.Q1[comp] : a.Q3[comp]) + tbase;
 #pragma unroll
-                for (int j1 = 0; j1 < G::N1; ++j1) dst[(long long)(jj + G::N2 * j1) * a.sx] = d[j1];
+                for (int j1 = 0; j1 < G::N1; ++j1) (dst + (long long)(G::N2 * j1) * sxi)[voff] = d[j1];
             }
         }
     }
@@ -680,7 +685,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
     constexpr int R = G::P / 8;
     constexpr int n = G::N;
     static_assert(R >= 1, "tile too small");
-    static_assert(G::PS >= n, "staging must fit in the pencil buffer");
+    static_assert(G::PS >= n + 1, "staging must fit in the pencil buffer");
     __shared__ float2 lds[8 * R * G::PS];
     __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
@@ -733,15 +738,16 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
         __syncthreads();
         if (act1) inv_lb<G>(d, pen1, j);
         __syncthreads();
-        // stage: st[field][rr][iz], field = 2*kind + part
+        // stage: st[field][rr][iz] with rows padded to n + 1 floats (odd), so
+        // the rows written by one wave fall in different banks, field = 2*kind + part
         float* st = reinterpret_cast<float*>(lds);
         if (act1) {
             const int rr = q1 / 8;
 #pragma unroll
             for (int j1 = 0; j1 < G::N1; ++j1) {
                 const int iz = j + G::N2 * j1;
-                st[((2 * kind) * R + rr) * n + iz] = d[j1].x;
-                st[((2 * kind + 1) * R + rr) * n + iz] = d[j1].y;
+                st[((2 * kind) * R + rr) * (n + 1) + iz] = d[j1].x;
+                st[((2 * kind + 1) * R + rr) * (n + 1) + iz] = d[j1].y;
             }
         }
         __syncthreads();
@@ -767,7 +773,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
         if (row >= a.nrows) continue;
         const int ix = (int)(row / a.ny), iy = (int)(row - (long long)ix * a.ny);
         const unsigned o = (unsigned)cell;  // == rr * n + iz
-        auto S = [&](int f) { return st[(f * R + rr) * n + iz]; };
+        auto S = [&](int f) { return st[(f * R + rr) * (n + 1) + iz]; };
         if (ix < a.nx - 1 && iy < a.ny - 1 && iz < n - 1) {
             // staging fields of the family members j = 0..4 (A1 v, v, A3 v, A1 dv, dv)
             const int fz[5] = {4, 10, 14, 5, 11};   // vz family (D+): normal dz vz
