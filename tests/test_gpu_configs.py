@@ -11,6 +11,7 @@ import os
 import numpy as np
 import pytest
 
+import oracle as O
 import qvti
 from cases import rel_l2
 
@@ -33,4 +34,67 @@ def test_config1_iso64_500_steps_vs_golden():
     e64 = rel_l2(rec[2], g["vz_f64"])
     floor = rel_l2(g["rec_f32"][2], g["vz_f64"])
     print("config 1 vz: vs FP32 oracle %.3e, vs FP64 oracle %.3e (FP32 oracle vs FP64 %.3e)" % (e32, e64, floor))
+    assert e32 < 1e-5 and e64 < 1e-5
+
+
+def _run_window(g, m, shot_kind, src, rx, ry, rz, nt):
+    """GPU records and FP32 / FP64 oracle records of the first nt steps."""
+    rik = qvti.ricker(nt, g["dt"], g["f0"])
+    with qvti.Engine(g["nx"], g["ny"], g["nz"], g["pml"], g["dx"], g["dy"], g["dz"], g["dt"], g["f0"], g["w0"],
+                     vti=g["vti"]) as e:
+        e.set_model_dict(m)
+        rec = e.forward(nt, rik, qvti.make_shot(shot_kind, src, rx, ry, rz))
+    shot = dict(nx=g["nx"], ny=g["ny"], nz=g["nz"], pml=g["pml"], nt=nt, dx=g["dx"], dy=g["dy"], dz=g["dz"],
+                dt=g["dt"], f0=g["f0"], w0=g["w0"], src_kind=shot_kind, sx=src[0], sy=src[1], sz=src[2],
+                rx=rx, ry=ry, rz=rz, rik=rik)
+    model = {k: m[k] for k in ("vp", "gp", "vs", "gs", "rho")}
+    if g["vti"]:
+        model["eps"], model["delta"] = m["eps"], m["delta"]
+    r32 = O.run(shot, model, "f32")["records"]
+    r64 = O.run(shot, model, "f64")["records"]
+    return rec, r32, r64
+
+
+def test_config2_vti128_layered_vs_golden():
+    """VTI 128^3, 6 seeded flat layers (vp U[1800,4500], vs = vp/1.73, Gardner
+    rho, eps U[0,0.25], delta U[0,eps], Qp U[30,200], Qs = Qp/2), dx 10 m,
+    20-cell C-PML, Ricker 20 Hz vertical force at (64,64,10), 128x128
+    receivers at z = 20, 200 steps of 0.5 ms (1 ms exceeds the scheme's
+    stability limit for these speeds), against the committed FP32/FP64 oracle
+    gathers (tests/golden/cfg2_records.npz, make_cfg2_golden.py; vz at every
+    4th receiver).  Far receivers the wave reaches last carry mostly the FP32
+    rounding floor, hence the floor-aware bound."""
+    gold = np.load(os.path.join(HERE, "golden", "cfg2_records.npz"))
+    n, pml, f0, nt, dt = 128, 20, 20.0, 200, 5e-4
+    m = qvti.make_model(1, 11, (n, n, n), [1800, 4500, 0.25, 30, 200], nlay=6, vti=True)
+    assert m["vp"].astype(np.float64).sum() == gold["vp_sum"]
+    rik = qvti.ricker(nt, dt, f0)
+    assert np.array_equal(rik, gold["rik"])
+    ix, iy = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    keep = ((ix % 4 == 0) & (iy % 4 == 0)).ravel()
+    with qvti.Engine(n, n, n, pml, 10.0, 10.0, 10.0, dt, f0, float(np.float32(2 * PI * f0)), vti=True) as e:
+        e.set_model_dict(m)
+        rec = e.forward(nt, rik, qvti.make_shot(qvti.SRC_FZ, (64, 64, 10), ix.ravel(), iy.ravel(),
+                                                np.full(n * n, pml)))
+    vz = rec[2][:, keep]
+    e32, e64 = rel_l2(vz, gold["vz_f32"]), rel_l2(vz, gold["vz_f64"])
+    floor = rel_l2(gold["vz_f32"], gold["vz_f64"])
+    print("config 2 vz: vs FP32 oracle %.3e, vs FP64 oracle %.3e (FP32 oracle vs FP64 %.3e)" % (e32, e64, floor))
+    assert e64 < max(1e-5, 2 * floor) and e32 < max(1e-5, 3 * floor)
+
+
+@pytest.mark.parametrize("workload,nt", [("vti256", 3), ("vti400", 2)])
+def test_config3_full_size_window(workload, nt):
+    """The 256^3 and 400x400x200 single-shot configurations (seeded smooth
+    random VTI model, Q in [20,150], eps/delta co-varying with vp, 20-cell
+    C-PML, dx 12.5 m, dt 1 ms): the first steps at full size against the
+    oracle, on a receiver line through the source."""
+    import bench
+
+    g, m, _ = bench.make_workload(workload)
+    src = (g["nx"] // 2, g["ny"] // 2, g["pml"] + 1)
+    rx = np.arange(src[0] - 6, src[0] + 7)
+    rec, r32, r64 = _run_window(g, m, qvti.SRC_FZ, src, rx, np.full(len(rx), src[1]), np.full(len(rx), src[2]), nt)
+    e32, e64 = rel_l2(rec, r32), rel_l2(rec, r64)
+    print("%s %d steps: vs FP32 oracle %.3e, vs FP64 oracle %.3e" % (workload, nt, e32, e64))
     assert e32 < 1e-5 and e64 < 1e-5

@@ -1,0 +1,117 @@
+"""GPU tests of the engine's other entry points and hooks, against the oracle:
+snapshots, the blow-up probe, the reference-compatible entry point
+(cuda_forward_acoustic_3D's argument list), and exact linearity of the whole
+path in the source at a full benchmark size."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+import qvti
+from cases import rel_l2, small_case
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def engine_for(shot, model, max_shots=1):
+    e = qvti.Engine(shot["nx"], shot["ny"], shot["nz"], shot["pml"], shot["dx"], shot["dy"], shot["dz"],
+                    shot["dt"], shot["f0"], shot["w0"], vti="eps" in model, max_shots=max_shots)
+    e.set_model_dict(model)
+    return e
+
+
+def shot_of(shot):
+    return qvti.make_shot(shot["src_kind"], (shot["sx"], shot["sy"], shot["sz"]), shot["rx"], shot["ry"], shot["rz"])
+
+
+@pytest.mark.parametrize("field,every", [(2, 7), (3, 5), (7, 9)])
+def test_snapshots_match_the_oracle(field, every):
+    """qvti_set_snapshot: interior window [pml, n-pml)^3 of one field every
+    `every` steps (it % every == 0, after the source), ix slowest - the
+    reference's disabled dump (cuviscoelas_3D.cu:1615-1627)."""
+    shot, model = small_case(24, 20, 32, 4, 30, O.SRC_EXPLOSIVE, True)
+    ref = O.run(shot, model, "f64", snap_every=every, snap_field=field)
+    with engine_for(shot, model) as e:
+        snaps = e.set_snapshot(field, every, shot["nt"])
+        e.begin(shot["nt"], shot["rik"], [shot_of(shot)])
+        e.step(shot["nt"])
+        e.finish()
+    want = ref["snaps"].reshape(snaps.shape)
+    assert snaps.shape[0] == (shot["nt"] + every - 1) // every
+    for k in range(1, snaps.shape[0]):  # snapshot 0 is the first step's field (tiny)
+        assert rel_l2(snaps[k], want[k]) < TOL, (k, rel_l2(snaps[k], want[k]))
+
+
+def test_blow_up_probe_reports_an_unstable_run():
+    """A time step far beyond the pseudospectral stability limit grows without
+    bound; the probe (max |v| every `every` steps) turns that into
+    QVTI_E_UNSTABLE instead of returning Inf/NaN seismograms."""
+    shot, model = small_case(16, 20, 24, 3, 200, O.SRC_FX, False, dt=8e-3)
+    with engine_for(shot, model) as e:
+        e.set_probe(10, 1e6)
+        with pytest.raises(qvti.QvtiError) as err:
+            e.forward(shot["nt"], shot["rik"], shot_of(shot))
+        assert err.value.code == qvti.E_UNSTABLE
+    shot, model = small_case(16, 20, 24, 3, 200, O.SRC_FX, False)
+    with engine_for(shot, model) as e:
+        e.set_probe(10, 1e6)
+        rec = e.forward(shot["nt"], shot["rik"], shot_of(shot))  # stable: no error
+        assert np.isfinite(rec).all()
+
+
+def test_reference_entry_point_matches_the_oracle():
+    """qvti_forward_acoustic_3D: the reference's argument list; x-force at
+    (s_ix, s_iy, s_iz + 10), receivers (r_ix[ir], r_iy[ir], r_iz) for
+    ir < rnx_max * rny_max, records [nt][R] (cuviscoelas_3D.cu:771-791)."""
+    shot, model = small_case(32, 24, 40, 4, 35, O.SRC_FX, False)
+    nx, ny, nz, pml, nt = shot["nx"], shot["ny"], shot["nz"], shot["pml"], shot["nt"]
+    rnx, rny, dr = 5, 4, 3
+    rix, riy = qvti.reference_receivers(pml, rnx, rny, dr)
+    ss = qvti.RefSource(8, nx // 2, ny // 2, pml + 2,
+                        rix.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                        riy.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), rnx * rny)
+    srcs = (qvti.RefSource * 1)(ss)
+    R = rnx * rny
+    recs = np.zeros((3, nt, R), np.float32)
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    m = [f32(model[k]) for k in ("vp", "gp", "vs", "gs", "rho")]
+    rik = f32(shot["rik"])
+    with qvti.Engine(nx, ny, nz, pml, shot["dx"], shot["dy"], shot["dz"], shot["dt"], shot["f0"], shot["w0"]) as e:
+        qvti.check(qvti.lib().qvti_forward_acoustic_3D(
+            e._c, 0, 0, nt, nx, ny, nz, nx * ny * nz, nx - 2 * pml, ny - 2 * pml, nz - 2 * pml, pml,
+            shot["dx"], shot["dy"], shot["dz"], shot["dt"], shot["f0"], shot["w0"], 4000.0,
+            rik.ctypes.data, *[a.ctypes.data for a in m], ctypes.cast(srcs, ctypes.c_void_p), R, rnx, rny, dr,
+            recs[0].ctypes.data, recs[1].ctypes.data, recs[2].ctypes.data))
+    oshot = dict(shot, src_kind=O.SRC_FX, sx=nx // 2, sy=ny // 2, sz=8 + 10, rx=rix, ry=riy,
+                 rz=np.full(R, pml + 2, np.int32))
+    ref = O.run(oshot, model, "f64")
+    for c in range(3):
+        assert rel_l2(recs[c], ref["records"][c]) < TOL, (c, rel_l2(recs[c], ref["records"][c]))
+
+
+@pytest.mark.parametrize("workload", ["vti256", "vti160"])
+def test_exact_linearity_in_the_source_at_full_size(workload):
+    """Scaling the wavelet by 2 scales every record bit-for-bit (every
+    operation of the step is linear in the source and a factor of 2 commutes
+    with FP32 rounding), at the benchmark's full 256^3 / 160x160x128 sizes; and
+    batched shots equal single shots bit-for-bit there."""
+    import bench
+
+    g, m, sh = bench.make_workload(workload)
+    nt = 12
+    rik = qvti.ricker(nt, g["dt"], g["f0"]) * np.float32(1e3)
+    shots = bench.surface_shots(g, 2, 3)
+    with bench.engine(g, max_shots=2) as e:
+        e.set_model_dict(m)
+        a = e.forward_batch(nt, rik, shots)
+        b = e.forward_batch(nt, rik * np.float32(2), shots)
+    with bench.engine(g) as e:
+        e.set_model_dict(m)
+        single = e.forward(nt, rik, shots[1])
+    assert np.abs(a[0]).max() > 0
+    for x, y in zip(a, b):
+        assert np.array_equal(2 * x, y)
+    assert np.array_equal(single, a[1])
