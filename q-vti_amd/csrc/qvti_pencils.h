@@ -39,6 +39,9 @@ constexpr int kThreads = 256;
 #ifndef QVTI_XPASS_WAVES
 #define QVTI_XPASS_WAVES 1
 #endif
+#ifndef QVTI_VZ_CHUNK
+#define QVTI_VZ_CHUNK 4
+#endif
 enum { MODE_STORE = 0, MODE_VEL_FIRST = 1, MODE_VEL_ACC = 2 };
 
 template <class G>
@@ -221,6 +224,62 @@ struct VelZArgs {
     const int* it;
 };
 
+// Velocity update of one component on the thread's level-1 elements of one
+// row (iz = j + N2*j1), from its z-derivative d (real or imaginary part):
+// C-PML memory variable on the z-slab, v += b * (acc + dz + phi), dv = v_new
+// - v_old (+ the previous step's source at the injection cell).  Writes v and
+// dv (rows with rowdom / ok1 only), returns (v_new, dv).  The pointers are
+// row bases (v, dv, acc, bu: the row; ph: the row's z-slab of 2*pml values)
+// and never alias, which lets the compiler issue the loads of later elements
+// before the stores of earlier ones.
+template <class G>
+__device__ __forceinline__ void vz_component(float2 (&out)[G::N1], const float2 (&d)[G::N1], const bool imag,
+                                             float* __restrict__ v, float* __restrict__ dv,
+                                             const float* __restrict__ acc, const float* __restrict__ bu,
+                                             float* __restrict__ ph, const float* __restrict__ pa,
+                                             const float* __restrict__ pb, const int j, const int pml,
+                                             const float dt, const bool rowdom, const bool ok1, const int src_iz,
+                                             const float sprev)
+{
+    constexpr int n = G::N;
+    // all loads and arithmetic first (every load unconditional and in bounds:
+    // ph is a valid slab row even where it is not used), all stores after, so
+    // no load waits behind a store or a branch
+    // (in chunks of QVTI_VZ_CHUNK elements, to bound the registers in flight)
+    constexpr int CH = G::N1 % QVTI_VZ_CHUNK == 0 ? QVTI_VZ_CHUNK : G::N1;
+#pragma unroll
+    for (int c0 = 0; c0 < G::N1; c0 += CH) {
+    float qn[CH];
+#pragma unroll
+    for (int jc = 0; jc < CH; ++jc) {
+        const int j1 = c0 + jc;
+        const int iz = j + G::N2 * j1;
+        const float vold = v[iz], bb = bu[iz], ac = acc[iz];
+        const bool slab = iz < pml || iz >= n - pml;
+        const int si = iz < pml ? iz : (iz >= n - pml ? iz - (n - 2 * pml) : 0);
+        const float q0 = ph[si], A = pa[iz], B = pb[iz];
+        const bool upd = rowdom && iz < n - 1;
+        float dz = imag ? d[j1].y : d[j1].x;
+        qn[jc] = q0 + dt * (-A * q0 - B * dz);
+        if (upd && slab) dz = dz + qn[jc];
+        const float vnew = upd ? vold + bb * (ac + dz) : vold;
+        float dvv = vnew - vold;
+        if (iz == src_iz) dvv += sprev;
+        out[j1] = make_float2(vnew, dvv);
+    }
+#pragma unroll
+    for (int jc = 0; jc < CH; ++jc) {
+        const int j1 = c0 + jc;
+        const int iz = j + G::N2 * j1;
+        const bool upd = rowdom && iz < n - 1;
+        if (upd && (iz < pml || iz >= n - pml)) ph[iz < pml ? iz : iz - (n - 2 * pml)] = qn[jc];
+        if (upd) v[iz] = out[j1].x;
+        if (ok1) dv[iz] = out[j1].y;
+    }
+    asm volatile("" ::: "memory");  // the next chunk's loads stay after this chunk
+    }
+}
+
 template <class G>
 __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
 {
@@ -276,32 +335,15 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
     const bool rowdom = ok1 && ix < a.nx - 1 && iy < a.ny - 1;
     const long long slab_row = (long long)shot * a.slab_stride + r1 * 2 * a.pml;
 
-    // velocity update of component c at element j1 from its z-derivative dz:
-    // writes v and dv, returns (v_new, dv)
-    auto update1 = [&](int c, int j1, float dz) {
-        const int iz = j + G::N2 * j1;
-        const unsigned o = lrow1 + iz;
-        const float vold = a.v[c][ubase + o];
-        float vnew = vold;
-        if (rowdom && iz < n - 1) {
-            if (iz < a.pml || iz >= n - a.pml) {
-                const int si = iz < a.pml ? iz : iz - (n - 2 * a.pml);
-                float* ph = a.phi[c] + slab_row + si;
-                const bool h = c == 2;
-                const float A = h ? a.pah[iz] : a.pa[iz];
-                const float B = h ? a.pbh[iz] : a.pb[iz];
-                const float q = *ph + a.dt * (-A * *ph - B * dz);
-                *ph = q;
-                dz = dz + q;
-            }
-            const float* bu = a.buoy[c] + (ubase - (long long)shot * a.shot_stride);
-            vnew = vold + bu[o] * (a.acc[c][ubase + o] + dz);
-            if (ok1) a.v[c][ubase + o] = vnew;
-        }
-        float dv = vnew - vold;
-        if (r1 * n + iz == sip && kind == c) dv += sprev;
-        if (ok1) a.dv[c][ubase + o] = dv;
-        return make_float2(vnew, dv);
+    // component c on this thread's elements of its row
+    auto update = [&](float2 (&out)[G::N1], const float2 (&d)[G::N1], int c, bool imag) {
+        const long long rb = ubase + lrow1;  // row base of this thread's row in the shot's fields
+        const int src_iz = (kind == c && sip >= r1 * n && sip < (r1 + 1) * n) ? (int)(sip - r1 * n) : -1;
+        const bool h = c == 2;
+        vz_component<G>(out, d, imag, a.v[c] + rb, a.dv[c] + rb, a.acc[c] + rb,
+                        a.buoy[c] + (rb - (long long)shot * a.shot_stride),
+                        a.phi[c] + (ok1 ? slab_row : (long long)shot * a.slab_stride),
+                        h ? a.pah : a.pa, h ? a.pbh : a.pb, j, a.pml, a.dt, rowdom, ok1, src_iz, sprev);
     };
     // forward z-FFT of x (first pass of the 3-D transform) into dst
     auto fft_store = [&](float2 (&x)[G::N1], float2* dstp) {
@@ -323,13 +365,21 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
     {
         float2 d[G::N1];
         deriv(a.pxz, a.pyz, a.symp, d);  // dz pxz (vx) + i dz pyz (vy), D+
+        float2 u[G::N1];
+        update(u, d, 0, false);
+        // park (vx, dvx) in this thread's own pencil slots while vy is updated
+        __syncthreads();  // every thread has read the pencil buffer (inv_lb)
+        if (act1) {  // (threads past the last pencil own no slots)
+#pragma unroll
+            for (int j1 = 0; j1 < G::N1; ++j1) pen1[j + G::N2 * j1] = u[j1];
+        }
+        update(u, d, 1, true);
         float2 x0[G::N1], x1[G::N1];
 #pragma unroll
         for (int j1 = 0; j1 < G::N1; ++j1) {
-            const float2 ux = update1(0, j1, d[j1].x);
-            const float2 uy = update1(1, j1, d[j1].y);
-            x0[j1] = make_float2(ux.x, uy.x);
-            x1[j1] = make_float2(ux.y, uy.y);
+            const float2 ux = act1 ? pen1[j + G::N2 * j1] : make_float2(0.f, 0.f);
+            x0[j1] = make_float2(ux.x, u[j1].x);
+            x1[j1] = make_float2(ux.y, u[j1].y);
         }
         fft_store(x0, a.P[0]);
         fft_store(x1, a.P[1]);
@@ -338,8 +388,7 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
         float2 d[G::N1];
         deriv(a.pzz, nullptr, a.symm, d);  // dz pzz (vz), D-
         float2 x2[G::N1];
-#pragma unroll
-        for (int j1 = 0; j1 < G::N1; ++j1) x2[j1] = update1(2, j1, d[j1].x);
+        update(x2, d, 2, false);
         fft_store(x2, a.P[2]);
     }
 }
