@@ -781,7 +781,10 @@ static void launch_class(qvti_ctx* c, int cls)
         });
         break;
     case KC_STR_Z:
-        dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_stress_z<decltype(g)>, dim3(c->g_sz.x, S), blk, 0, st, c->a_sz); });
+        dispatch_n(nz, [&](auto g) {
+            if (c->g.vti) hipLaunchKernelGGL((k2_stress_z<decltype(g), true>), dim3(c->g_sz.x, S), blk, 0, st, c->a_sz);
+            else hipLaunchKernelGGL((k2_stress_z<decltype(g), false>), dim3(c->g_sz.x, S), blk, 0, st, c->a_sz);
+        });
         break;
     case KC_RECORD:
         hipLaunchKernelGGL(k_record, dim3(c->g_rec.x, S), blk, 0, st, c->a_rec);

@@ -728,7 +728,7 @@ struct StressZArgs {
 //   0: A1 vx + i A1 dvx (D-)   1: A1 vy + i A1 dvy (D-)   2: A1 vz + i A1 dvz (D+)
 //   3: vx + i dvx (D-)         4: vy + i dvy (D-)         5: vz + i dvz (D+)
 //   6: A3 vx + i A3 vy (D-)    7: A3 vz (D+)
-template <class G>
+template <class G, bool VTI>
 __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
 {
     constexpr int R = G::P / 8;
@@ -812,7 +812,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
     const long long fbase = soff + cbase;    // per-shot natural fields
     const float* __restrict__ dxs = a.dxb + (long long)shot * kG1 * a.shot_stride;
     const float* __restrict__ dys = a.dyb + (long long)shot * kG1 * a.shot_stride;
-    const bool vti = a.g2 != nullptr;
+    constexpr bool vti = VTI;  // G2 (H, D sets) present: a compile-time choice keeps the loads branch-free
     const long long sxb = (long long)shot * a.slabx_stride, syb = (long long)shot * a.slaby_stride,
                     szb = (long long)shot * a.slabz_stride;
 #pragma unroll 1
