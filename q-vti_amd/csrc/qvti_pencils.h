@@ -42,6 +42,9 @@ constexpr int kThreads = 256;
 #ifndef QVTI_VZ_CHUNK
 #define QVTI_VZ_CHUNK 4
 #endif
+#ifndef QVTI_VEL_CHUNK
+#define QVTI_VEL_CHUNK 4
+#endif
 enum { MODE_STORE = 0, MODE_VEL_FIRST = 1, MODE_VEL_ACC = 2 };
 
 template <class G>
@@ -166,29 +169,55 @@ void k2_deriv_strided(StridedArgs a)
 #pragma unroll
                 for (int j1 = 0; j1 < G::N1; ++j1) pen[jj + G::N2 * j1] = d[j1];
                 const int pml = a.pml, n = G::N;
-#pragma unroll 2
-                for (int j1 = 0; j1 < G::N1; ++j1) {
-                    const int i = jj + G::N2 * j1;
-                    const long long urow = ubase + (long long)(G::N2 * j1) * astride;
-                    float2 r = pen[i];
-                    const int ix = a.axis == 0 ? i : other;
-                    const int iy = a.axis == 0 ? other : i;
-                    if ((i < pml || i >= n - pml) && ix < a.nx - 1 && iy < a.ny - 1) {
-                        const int si = i < pml ? i : i - (n - 2 * pml);
-                        const int sidx = a.axis == 0 ? (si * a.ny + iy) * nz + zc : (ix * 2 * pml + si) * nz + zc;
-                        float2* ph = reinterpret_cast<float2*>(J.phi + (long long)shot * a.slab_stride) + sidx / 2;
-                        const float A = J.half ? a.pah[i] : a.pa[i];
-                        const float B = J.half ? a.pbh[i] : a.pb[i];
-                        float2 q = *ph;
-                        if (zc < nz - 1) { q.x = q.x + a.dt * (-A * q.x - B * r.x); r.x = r.x + q.x; }
-                        if (zc + 1 < nz - 1) { q.y = q.y + a.dt * (-A * q.y - B * r.y); r.y = r.y + q.y; }
-                        *ph = q;
+                // an opaque copy of the lane's row index: keeps the per-element
+                // C-PML indices and table loads inside the job loop instead of
+                // being hoisted (and held in registers through the FFTs)
+                int jl = jj;
+                asm volatile("" : "+v"(jl));
+                const float* __restrict__ pa = J.half ? a.pah : a.pa;
+                const float* __restrict__ pb = J.half ? a.pbh : a.pb;
+                float2* __restrict__ phs = reinterpret_cast<float2*>(J.phi + (long long)shot * a.slab_stride);
+                // Chunks of CH elements: every load of a chunk (unconditional
+                // and in bounds - a non-slab element reads slab plane 0) before
+                // its stores, so the chunk's memory round trips overlap.
+                constexpr int CH = G::N1 % QVTI_VEL_CHUNK == 0 ? QVTI_VEL_CHUNK : 1;
+#pragma unroll
+                for (int c0 = 0; c0 < G::N1; c0 += CH) {
+                    float2 rr[CH], q[CH];
+                    int sidx[CH];
+                    bool upd[CH];
+#pragma unroll
+                    for (int jc = 0; jc < CH; ++jc) {
+                        const int j1 = c0 + jc;
+                        const int i = jl + G::N2 * j1;
+                        const long long urow = ubase + (long long)(G::N2 * j1) * astride;
+                        float2 r = pen[i];
+                        const int ix = a.axis == 0 ? i : other;
+                        const int iy = a.axis == 0 ? other : i;
+                        upd[jc] = (i < pml || i >= n - pml) && ix < a.nx - 1 && iy < a.ny - 1;
+                        const int si = i < pml ? i : (i >= n - pml ? i - (n - 2 * pml) : 0);
+                        sidx[jc] = (a.axis == 0 ? (si * a.ny + iy) * nz + zc : (ix * 2 * pml + si) * nz + zc) / 2;
+                        const float2 q0 = phs[sidx[jc]];
+                        const float A = pa[i], B = pb[i];
+                        q[jc] = q0;
+                        if (upd[jc]) {
+                            if (zc < nz - 1) { q[jc].x = q0.x + a.dt * (-A * q0.x - B * r.x); r.x = r.x + q[jc].x; }
+                            if (zc + 1 < nz - 1) { q[jc].y = q0.y + a.dt * (-A * q0.y - B * r.y); r.y = r.y + q[jc].y; }
+                        }
+                        if (MODE == MODE_VEL_ACC) {
+                            const float2 ac = *reinterpret_cast<const float2*>(J.acc + urow + voff);
+                            r = make_float2(ac.x + r.x, ac.y + r.y);
+                        }
+                        rr[jc] = r;
                     }
-                    if (MODE == MODE_VEL_ACC) {
-                        const float2 ac = *reinterpret_cast<const float2*>(J.acc + urow + voff);
-                        r = make_float2(ac.x + r.x, ac.y + r.y);
+#pragma unroll
+                    for (int jc = 0; jc < CH; ++jc) {
+                        const int j1 = c0 + jc;
+                        const long long urow = ubase + (long long)(G::N2 * j1) * astride;
+                        if (upd[jc]) phs[sidx[jc]] = q[jc];
+                        *reinterpret_cast<float2*>(J.out + urow + voff) = rr[jc];
                     }
-                    *reinterpret_cast<float2*>(J.out + urow + voff) = r;
+                    asm volatile("" ::: "memory");  // the next chunk's loads stay after this chunk
                 }
             }
         }
@@ -242,6 +271,8 @@ __device__ __forceinline__ void vz_component(float2 (&out)[G::N1], const float2 
                                              const float sprev)
 {
     constexpr int n = G::N;
+    int jl = j;  // opaque per call: nothing is shared (and held) across components
+    asm volatile("" : "+v"(jl));
     // all loads and arithmetic first (every load unconditional and in bounds:
     // ph is a valid slab row even where it is not used), all stores after, so
     // no load waits behind a store or a branch
@@ -253,7 +284,7 @@ __device__ __forceinline__ void vz_component(float2 (&out)[G::N1], const float2 
 #pragma unroll
     for (int jc = 0; jc < CH; ++jc) {
         const int j1 = c0 + jc;
-        const int iz = j + G::N2 * j1;
+        const int iz = jl + G::N2 * j1;
         const float vold = v[iz], bb = bu[iz], ac = acc[iz];
         const bool slab = iz < pml || iz >= n - pml;
         const int si = iz < pml ? iz : (iz >= n - pml ? iz - (n - 2 * pml) : 0);
@@ -270,7 +301,7 @@ __device__ __forceinline__ void vz_component(float2 (&out)[G::N1], const float2 
 #pragma unroll
     for (int jc = 0; jc < CH; ++jc) {
         const int j1 = c0 + jc;
-        const int iz = j + G::N2 * j1;
+        const int iz = jl + G::N2 * j1;
         const bool upd = rowdom && iz < n - 1;
         if (upd && (iz < pml || iz >= n - pml)) ph[iz < pml ? iz : iz - (n - 2 * pml)] = qn[jc];
         if (upd) v[iz] = out[j1].x;
@@ -516,6 +547,8 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
         float2 b[G::N2];
         if (act2) fwd_l2<G>(b, pen, jj);
         const int nops = a.Q3[comp] ? 2 : 1;  // |k| (op 1) only where its output is used
+        int jl = jj;  // opaque copy: the |k| symbols are computed here, not hoisted and held
+        asm volatile("" : "+v"(jl));
 #pragma unroll
         for (int op = 0; op < 2; ++op) {
             if (op >= nops) break;
@@ -524,7 +557,7 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
                 float2 c[G::N2];
 #pragma unroll
                 for (int k2 = 0; k2 < G::N2; ++k2) {
-                    const double kx = a.kfx[jj + G::N1 * k2];
+                    const double kx = a.kfx[jl + G::N1 * k2];
                     const double k2s = kx * kx + kyz;
                     // 1/sqrt(k2s) from the hardware estimate and two Newton
                     // steps (relative error far below 2^-50, so the FP32
