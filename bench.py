@@ -178,12 +178,12 @@ def time_steps(e, nt, rik, shots, steps, warmup):
     return time.perf_counter() - t0
 
 
-def run_batched(steps, warmup, nshots=32, nt_full=2000, widths=(8, 16, 32)):
-    """32 shots of one 160x160x128 VTI model: the plain path (one shot per
-    launch sequence) against batched launches (W shots per kernel launch,
-    W picked by timing each candidate).  shots/s is for nt_full-step shots,
-    from the timed per-step wall time of each path."""
-    g, m, _ = make_workload("vti160")
+def run_batched(steps, warmup, nshots=32, nt_full=2000, widths=(8, 16, 32), workload="vti160"):
+    """32 shots of one model (default: the 160x160x128 VTI workload): the
+    plain path (one shot per launch sequence) against batched launches (W
+    shots per kernel launch, W picked by timing each candidate).  shots/s is
+    for nt_full-step shots, from the timed per-step wall time of each path."""
+    g, m, _ = make_workload(workload)
     shots = surface_shots(g, nshots, 11)
     nt = warmup + steps
     rik = qvti.ricker(nt, g["dt"], g["f0"])
@@ -204,13 +204,16 @@ def run_batched(steps, warmup, nshots=32, nt_full=2000, widths=(8, 16, 32)):
     N = g["nx"] * g["ny"] * g["nz"]
     plain_sps = 1.0 / (plain_step * nt_full)
     batched_sps = 1.0 / (per_shot_step * nt_full)
+    wl = "%dx%dx%d %s" % (g["nx"], g["ny"], g["nz"], "VTI" if g["vti"] else "isotropic")
     return {
-        "metric": "batched shots/s (160x160x128 VTI, 2000-step shots)",
+        "metric": "batched shots/s (%s, %d-step shots)" % (wl, nt_full),
         "value": round(batched_sps, 4), "unit": "shots/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(per_shot_step * wbest * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded smooth random VTI model)",
-        "config": {"workload": "32 shots on one 160x160x128 VTI model, pml 20, dx 12.5 m, dt 1 ms, sources "
-                               "seeded on the surface grid", "cells": N, "parallelism": "1 GPU, %d shots per launch" % wbest},
+        "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded %s model)" % ("smooth random VTI" if g["vti"] else "homogeneous isotropic"),
+        "config": {"workload": "%d shots on one %s model, pml %d, dx %.1f m, dt %.0e s, sources seeded on the "
+                               "surface grid" % (nshots, wl, g["pml"], g["dx"], g["dt"]),
+                   "cells": N, "parallelism": "1 GPU, %d shots per launch" % wbest},
         "plain_shots_per_s": round(plain_sps, 4),
         "speedup_vs_plain": round(batched_sps / plain_sps, 3),
         "gcell_steps_per_s": {"plain": round(N / plain_step / 1e9, 3), "batched": round(N / per_shot_step / 1e9, 3)},
@@ -332,7 +335,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="vti256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--batched", action="store_true", help="32-shot 160x160x128 batched leg instead")
+    ap.add_argument("--batched", action="store_true", help="32-shot batched leg instead (--workload, default vti160)")
     ap.add_argument("--served", type=int, default=0, metavar="NREQ", help="serving leg with NREQ requests instead")
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -344,7 +347,9 @@ def main():
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.batched:
-        print(json.dumps(run_batched(args.steps, args.warmup)), flush=True)
+        wl = args.workload if args.workload != "vti256" else "vti160"
+        nt_full = 500 if wl == "iso64" else (3000 if wl in ("vti256", "vti400") else 2000)
+        print(json.dumps(run_batched(args.steps, args.warmup, workload=wl, nt_full=nt_full)), flush=True)
         return
     if args.served:
         print(json.dumps(run_served(args.served)), flush=True)
