@@ -534,6 +534,8 @@ static int build_launch_config(qvti_ctx* c)
     vz.rik = nullptr;  // set at begin
     vz.it = c->d_it;
     c->g_vz = dim3((unsigned)((c->nrows + Pz - 1) / Pz), 1, 1);
+    vz.ntiles = (int)c->g_vz.x;
+    vz.nshots = 1;
     c->g_zr = c->g_vz;
 
     // H: [A1 v x,y,z][A1 dv x,y,z][A3 v x,y,z]
@@ -649,6 +651,8 @@ static int build_launch_config(qvti_ctx* c)
     sz.it = c->d_it;
     const int R = Pz / 8;
     c->g_sz = dim3((unsigned)((c->nrows + R - 1) / R), 1, 1);
+    sz.ntiles = (int)c->g_sz.x;
+    sz.nshots = 1;
 
     RecArgs& rec = c->a_rec;
     rec.vx = vv[0]; rec.vy = vv[1]; rec.vz = vv[2];
@@ -721,6 +725,12 @@ static int record_event(qvti_ctx* c, int cls)
 
 // Enqueue one time step (14 launches).  With timing on, each launch is
 // bracketed by an event pair.
+// 1-D grid of the XCD-aware row kernels (tile_shot): ceil(ntiles/8)*8*S.
+static unsigned xcd_grid(int ntiles, unsigned S)
+{
+    return (unsigned)((ntiles + 7) / 8) * 8u * S;
+}
+
 // Launch one kernel of class cls on the context's stream.
 static void launch_class(qvti_ctx* c, int cls)
 {
@@ -745,7 +755,9 @@ static void launch_class(qvti_ctx* c, int cls)
         });
         break;
     case KC_VEL_Z:
-        dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_vel_z<decltype(g)>, dim3(c->g_vz.x, S), blk, 0, st, c->a_vz); });
+        dispatch_n(nz, [&](auto g) {
+            hipLaunchKernelGGL(k2_vel_z<decltype(g)>, dim3(xcd_grid(c->a_vz.ntiles, S)), blk, 0, st, c->a_vz);
+        });
         break;
     case KC_Y_FWD:
         dispatch_nw(ny, c->pw[cls], [&](auto g) {
@@ -782,8 +794,9 @@ static void launch_class(qvti_ctx* c, int cls)
         break;
     case KC_STR_Z:
         dispatch_n(nz, [&](auto g) {
-            if (c->g.vti) hipLaunchKernelGGL((k2_stress_z<decltype(g), true>), dim3(c->g_sz.x, S), blk, 0, st, c->a_sz);
-            else hipLaunchKernelGGL((k2_stress_z<decltype(g), false>), dim3(c->g_sz.x, S), blk, 0, st, c->a_sz);
+            const dim3 grid(xcd_grid(c->a_sz.ntiles, S));
+            if (c->g.vti) hipLaunchKernelGGL((k2_stress_z<decltype(g), true>), grid, blk, 0, st, c->a_sz);
+            else hipLaunchKernelGGL((k2_stress_z<decltype(g), false>), grid, blk, 0, st, c->a_sz);
         });
         break;
     case KC_RECORD:
@@ -1014,6 +1027,8 @@ extern "C" int qvti_begin(qvti_ctx* c, int nt, const float* rik, const qvti_shot
         }
     }
     // per-run kernel arguments
+    c->a_vz.nshots = nshots;
+    c->a_sz.nshots = nshots;
     c->a_vz.rik = c->d_rik;
     c->a_sz.rik = c->d_rik;
     c->a_rec.rec_ip = c->d_rec_ip;

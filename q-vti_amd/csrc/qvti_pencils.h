@@ -47,6 +47,23 @@ constexpr int kThreads = 256;
 #endif
 enum { MODE_STORE = 0, MODE_VEL_FIRST = 1, MODE_VEL_ACC = 2 };
 
+// XCD-aware placement of (row tile, shot) on a 1-D grid of
+// ceil(ntiles/8)*8*nshots workgroups.  Workgroup ids are dealt round-robin to
+// the 8 XCDs, so the shots of one row tile get ids that differ by multiples of
+// 8: they run on the same XCD (same L2), dispatched close together, and the
+// model fields they all read (coefficients, buoyancies) come from cache after
+// the first shot.  Returns tile = -1 for the padding workgroups.
+struct TileShot {
+    int tile, shot;
+};
+__device__ __forceinline__ TileShot tile_shot(int ntiles, int nshots)
+{
+    const int id = blockIdx.x, grp = 8 * nshots;
+    const int g = id / grp, r = id - g * grp;
+    const int t = g * 8 + (r & 7);
+    return {t < ntiles ? t : -1, r >> 3};
+}
+
 template <class G>
 __device__ __forceinline__ void load_twiddles(float2* tws, const float2* tw)
 {
@@ -234,6 +251,7 @@ void k2_deriv_strided(StridedArgs a)
 // Tile: G::P rows (z-pencils).
 struct VelZArgs {
     int nx, ny, nz, pml;
+    int ntiles, nshots;  // grid: tile_shot()
     long long nrows;
     float dt;
     long long shot_stride, slab_stride;
@@ -317,12 +335,14 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
     constexpr int n = G::N;
-    const int tid = threadIdx.x, shot = blockIdx.y;
+    const TileShot ts = tile_shot(a.ntiles, a.nshots);
+    if (ts.tile < 0) return;  // padding workgroup (whole workgroup, before any barrier)
+    const int tid = threadIdx.x, shot = ts.shot;
     // level 1/B mapping (column j fastest) and level 2/A mapping (row k1 fastest)
     const int j = tid % G::N2, p1 = tid / G::N2;
     const int k1 = tid % G::N1, p2 = tid / G::N1;
     const bool act1 = p1 < G::P, act2 = p2 < G::P;
-    const long long r0 = (long long)blockIdx.x * G::P;
+    const long long r0 = (long long)ts.tile * G::P;
     const long long r1 = r0 + p1, r2 = r0 + p2;
     const bool ok1 = act1 && r1 < a.nrows, ok2 = act2 && r2 < a.nrows;
     // uniform block base (shot + first row) and unsigned 32-bit lane offsets;
@@ -731,6 +751,7 @@ __global__ __launch_bounds__(kThreads) void k2_deriv_rows(RowDerivArgs a)
 // are staged in LDS, then every thread updates R*nz/256 cells.
 struct StressZArgs {
     int nx, ny, nz, pml;
+    int ntiles, nshots;  // grid: tile_shot()
     long long nrows;
     float dt;
     long long shot_stride, slabx_stride, slaby_stride, slabz_stride;
@@ -770,11 +791,13 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
     static_assert(G::PS >= n + 1, "staging must fit in the pencil buffer");
     __shared__ float2 lds[8 * R * G::PS];
     __shared__ float2 tws[G::TWN];
-    const int tid = threadIdx.x, shot = blockIdx.y;
+    const TileShot ts = tile_shot(a.ntiles, a.nshots);
+    if (ts.tile < 0) return;  // padding workgroup (whole workgroup, before any barrier)
+    const int tid = threadIdx.x, shot = ts.shot;
     const int j = tid % G::N2, q1 = tid / G::N2;
     const int k1 = tid % G::N1, q2 = tid / G::N1;
     const bool act1 = q1 < 8 * R, act2 = q2 < 8 * R;
-    const long long r0 = (long long)blockIdx.x * R;
+    const long long r0 = (long long)ts.tile * R;
     const long long row1 = r0 + q1 / 8;
     const bool ok1 = act1 && row1 < a.nrows;
     const long long soff = (long long)shot * a.shot_stride;

@@ -87,9 +87,14 @@ def test_reference_entry_point_matches_the_oracle():
             recs[0].ctypes.data, recs[1].ctypes.data, recs[2].ctypes.data))
     oshot = dict(shot, src_kind=O.SRC_FX, sx=nx // 2, sy=ny // 2, sz=8 + 10, rx=rix, ry=riy,
                  rz=np.full(R, pml + 2, np.int32))
-    ref = O.run(oshot, model, "f64")
+    ref = O.run(oshot, model, "f64")["records"]
+    r32 = O.run(oshot, model, "f32")["records"]
     for c in range(3):
-        assert rel_l2(recs[c], ref["records"][c]) < TOL, (c, rel_l2(recs[c], ref["records"][c]))
+        # vy and vz of an x-force are weak at these receivers: floor-aware bound
+        floor = rel_l2(r32[c], ref[c])
+        e = rel_l2(recs[c], ref[c])
+        print("component %d: vs FP64 %.2e (FP32 oracle floor %.2e)" % (c, e, floor))
+        assert e < max(TOL, 2 * floor), (c, e, floor)
 
 
 @pytest.mark.parametrize("workload", ["vti256", "vti160"])
