@@ -725,6 +725,12 @@ static int record_event(qvti_ctx* c, int cls)
 
 // Enqueue one time step (14 launches).  With timing on, each launch is
 // bracketed by an event pair.
+// Workgroups per row tile of the final inverse z pass (1: all fields in one
+// workgroup; 5: one field each).
+#ifndef QVTI_ZINV_SPLIT
+#define QVTI_ZINV_SPLIT 5
+#endif
+
 // 1-D grid of the XCD-aware row kernels (tile_shot): ceil(ntiles/8)*8*S.
 static unsigned xcd_grid(int ntiles, unsigned S)
 {
@@ -778,7 +784,9 @@ static void launch_class(qvti_ctx* c, int cls)
         });
         break;
     case KC_Z_INV:
-        dispatch_n(nz, [&](auto g) { hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3(c->g_zr.x, S), blk, 0, st, c->a_zi); });
+        dispatch_n(nz, [&](auto g) {
+            hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3(c->g_zr.x, S, QVTI_ZINV_SPLIT), blk, 0, st, c->a_zi);
+        });
         break;
     case KC_STR_X:
         dispatch_nw(nx, c->pw[cls], [&](auto g) {

@@ -637,7 +637,7 @@ __global__ __launch_bounds__(kThreads) void k2_zrows(ZRowsArgs a)
     float2* pen1 = lds + p1 * G::PS;
     float2* pen2 = lds + p2 * G::PS;
     load_twiddles<G>(tws, a.tw);
-    for (int f = 0; f < a.nfields; ++f) {
+    for (int f = blockIdx.z; f < a.nfields; f += gridDim.z) {
         __syncthreads();
         if (a.mode == 2 && a.dir < 0) {
             if (ok1) {
