@@ -180,6 +180,18 @@ int qvti_gamma_from_q(const float *q, float *gamma, long long n);
  * (viscoelas_3D.cpp:181-185). */
 int qvti_reference_receivers(int pml, int rnx_max, int rny_max, int dr, int *r_ix, int *r_iy);
 
+/* ---- data formats on either side of the path (host only) */
+/* Edge-replicate an interior model [(ntx-2pml)][(nty-2pml)][(ntz-2pml)] into
+ * the total grid [ntx][nty][ntz]: the reference's padding of its model files
+ * into the C-PML frame (viscoelas_3D.cpp:576-655). */
+int qvti_pad_model(const float *interior, int ntx, int nty, int ntz, int pml, float *total);
+/* Read an interior model from a raw FP32 file (ix slowest, the reference's
+ * input format, viscoelas_3D.cpp:541-546) and pad it into total. */
+int qvti_load_model_raw(const char *path, int ntx, int nty, int ntz, int pml, float *total);
+/* Write n FP32 values raw: record gathers [nt][R] as the reference dumps them
+ * (viscoelas_3D.cpp:416-434) or interior snapshots (cuviscoelas_3D.cu:1615-1627). */
+int qvti_write_raw(const char *path, const float *data, long long n);
+
 /* Seeded synthetic models for the benchmark configurations (splitmix64-seeded
  * xoshiro256**, u = (x >> 11) * 2^-53).  Outputs FP32 [nx][ny][nz]; eps/delta
  * may be NULL.  Q is returned through qp/qs (gamma via qvti_gamma_from_q).

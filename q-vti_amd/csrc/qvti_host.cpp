@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <thread>
 #include <vector>
 
@@ -192,4 +193,60 @@ extern "C" int qvti_make_model(int kind, uint64_t seed, int nx, int ny, int nz, 
         return QVTI_OK;
     }
     return qvti_fail(QVTI_E_ARG, "unknown model kind %d", kind);
+}
+
+// ---- data formats on either side of the path --------------------------------
+
+// Edge replication of an interior model into the total grid (C-PML frame):
+// viscoelas_3D.cpp:576-655 copies the first/last interior plane outward along
+// z, then y, then x, which equals reading the interior at the clamped index.
+extern "C" int qvti_pad_model(const float* interior, int ntx, int nty, int ntz, int pml, float* total)
+{
+    if (!interior || !total) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    const int wx = ntx - 2 * pml, wy = nty - 2 * pml, wz = ntz - 2 * pml;
+    if (pml < 0 || wx < 1 || wy < 1 || wz < 1) return qvti_fail(QVTI_E_ARG, "grid too small for pml %d", pml);
+    auto clampw = [](int i, int lo, int w) { return std::min(std::max(i - lo, 0), w - 1); };
+    parallel_for(ntx, [&](long long ix) {
+        const long long sx = clampw((int)ix, pml, wx);
+        for (int iy = 0; iy < nty; ++iy) {
+            const long long sy = clampw(iy, pml, wy);
+            const float* src = interior + (sx * wy + sy) * wz;
+            float* dst = total + (ix * nty + iy) * (long long)ntz;
+            for (int iz = 0; iz < ntz; ++iz) dst[iz] = src[clampw(iz, pml, wz)];
+        }
+    });
+    return QVTI_OK;
+}
+
+// Read an interior model (raw FP32, ix slowest, (ntx-2pml)(nty-2pml)(ntz-2pml)
+// values - the reference's input files, viscoelas_3D.cpp:541-546) and pad it
+// into the total grid.
+extern "C" int qvti_load_model_raw(const char* path, int ntx, int nty, int ntz, int pml, float* total)
+{
+    if (!path || !total) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    const long long wx = ntx - 2LL * pml, wy = nty - 2LL * pml, wz = ntz - 2LL * pml;
+    if (pml < 0 || wx < 1 || wy < 1 || wz < 1) return qvti_fail(QVTI_E_ARG, "grid too small for pml %d", pml);
+    const long long n = wx * wy * wz;
+    std::vector<float> in(n);
+    FILE* f = fopen(path, "rb");
+    if (!f) return qvti_fail(QVTI_E_ARG, "cannot open %s", path);
+    const size_t got = fread(in.data(), sizeof(float), (size_t)n, f);
+    fclose(f);
+    if ((long long)got != n)
+        return qvti_fail(QVTI_E_ARG, "%s holds %zu floats, the interior needs %lld", path, got, n);
+    return qvti_pad_model(in.data(), ntx, nty, ntz, pml, total);
+}
+
+// Write n FP32 values raw: record gathers [nt][R] as the reference dumps them
+// (viscoelas_3D.cpp:416-434), or interior snapshots [window] (ix slowest,
+// cuviscoelas_3D.cu:1615-1627).
+extern "C" int qvti_write_raw(const char* path, const float* data, long long n)
+{
+    if (!path || (!data && n > 0) || n < 0) return qvti_fail(QVTI_E_ARG, "bad arguments");
+    FILE* f = fopen(path, "wb");
+    if (!f) return qvti_fail(QVTI_E_ARG, "cannot create %s", path);
+    const size_t put = n ? fwrite(data, sizeof(float), (size_t)n, f) : 0;
+    const int closed = fclose(f);
+    if ((long long)put != n || closed != 0) return qvti_fail(QVTI_E_ARG, "short write to %s", path);
+    return QVTI_OK;
 }

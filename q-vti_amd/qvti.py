@@ -27,6 +27,7 @@ EXPORTS = [
     "qvti_ricker", "qvti_gamma_from_q", "qvti_reference_receivers", "qvti_make_model",
     "qvti_server_create", "qvti_server_submit", "qvti_server_poll", "qvti_server_wait",
     "qvti_server_cancel", "qvti_server_clock", "qvti_server_destroy", "qvti_set_autotune", "qvti_tile_choice",
+    "qvti_pad_model", "qvti_load_model_raw", "qvti_write_raw",
 ]
 
 
@@ -96,6 +97,8 @@ def lib():
             "qvti_server_poll": [vp, ctypes.c_uint64, vp, vp, vp, vp],
             "qvti_server_wait": [vp, ctypes.c_uint64, d], "qvti_server_cancel": [vp, ctypes.c_uint64],
             "qvti_server_destroy": [vp], "qvti_set_autotune": [vp, i], "qvti_tile_choice": [vp, i, vp, vp],
+            "qvti_pad_model": [vp, i, i, i, i, vp], "qvti_load_model_raw": [ctypes.c_char_p, i, i, i, i, vp],
+            "qvti_write_raw": [ctypes.c_char_p, vp, ll],
         }
         for name, args in sig.items():
             if not hasattr(L, name):
@@ -191,6 +194,26 @@ def deriv1d(a, d, axis, sym):
     out = np.zeros_like(a)
     check(lib().qvti_deriv1d(a.shape[0], a.shape[1], a.shape[2], d, axis, sym, _p(a), _p(out)))
     return out
+
+
+def pad_model(interior, pml):
+    """Edge-replicate an interior model (wx, wy, wz) into the total grid."""
+    a = f32(interior)
+    out = np.zeros(tuple(n + 2 * pml for n in a.shape), np.float32)
+    check(lib().qvti_pad_model(_p(a), out.shape[0], out.shape[1], out.shape[2], pml, _p(out)))
+    return out
+
+
+def load_model_raw(path, shape, pml):
+    """Interior raw FP32 file -> padded total-grid model of `shape`."""
+    out = np.zeros(shape, np.float32)
+    check(lib().qvti_load_model_raw(os.fsencode(path), shape[0], shape[1], shape[2], pml, _p(out)))
+    return out
+
+
+def write_raw(path, data):
+    a = f32(data)
+    check(lib().qvti_write_raw(os.fsencode(path), _p(a), a.size))
 
 
 def make_shot(kind, src, rx, ry, rz):

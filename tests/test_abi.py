@@ -97,3 +97,53 @@ def test_estimate_bytes_scales_with_shots():
     one = qvti.lib().qvti_estimate_bytes(ctypes.byref(g), 1)
     four = qvti.lib().qvti_estimate_bytes(ctypes.byref(g), 4)
     assert 3e9 < one < 12e9 and four > 2.5 * one
+
+
+def _reference_padding(interior, pml):
+    """The reference's padding loops, restated literally (viscoelas_3D.cpp:576-655):
+    z for interior x, y; then y for interior x, all z; then x for all y, z."""
+    wx, wy, wz = interior.shape
+    ntx, nty, ntz = wx + 2 * pml, wy + 2 * pml, wz + 2 * pml
+    v = np.zeros((ntx, nty, ntz), np.float32)
+    v[pml:ntx - pml, pml:nty - pml, pml:ntz - pml] = interior
+    for ix in range(pml, ntx - pml):
+        for iy in range(pml, nty - pml):
+            for iz in range(pml):
+                v[ix, iy, iz] = v[ix, iy, pml]
+            for iz in range(ntz - pml, ntz):
+                v[ix, iy, iz] = v[ix, iy, ntz - pml - 1]
+    for iz in range(ntz):
+        for ix in range(pml, ntx - pml):
+            for iy in range(pml):
+                v[ix, iy, iz] = v[ix, pml, iz]
+            for iy in range(nty - pml, nty):
+                v[ix, iy, iz] = v[ix, nty - pml - 1, iz]
+    for iz in range(ntz):
+        for iy in range(nty):
+            for ix in range(pml):
+                v[ix, iy, iz] = v[pml, iy, iz]
+            for ix in range(ntx - pml, ntx):
+                v[ix, iy, iz] = v[ntx - pml - 1, iy, iz]
+    return v
+
+
+def test_model_padding_matches_the_reference_loops():
+    rng = np.random.default_rng(4)
+    for shape, pml in (((5, 7, 6), 3), ((4, 4, 9), 2), ((1, 2, 3), 4)):
+        interior = rng.random(shape).astype(np.float32)
+        assert np.array_equal(qvti.pad_model(interior, pml), _reference_padding(interior, pml)), (shape, pml)
+
+
+def test_raw_model_and_record_files_round_trip(tmp_path):
+    rng = np.random.default_rng(5)
+    interior = rng.random((6, 5, 7)).astype(np.float32)
+    path = str(tmp_path / "vp.bin")
+    interior.tofile(path)  # the reference's input format: raw FP32, ix slowest
+    total = qvti.load_model_raw(path, (12, 11, 13), 3)
+    assert np.array_equal(total, _reference_padding(interior, 3))
+    with pytest.raises(qvti.QvtiError):
+        qvti.load_model_raw(path, (14, 11, 13), 3)  # file too short for that interior
+    rec = rng.random((40, 9)).astype(np.float32)  # [nt][R]
+    out = str(tmp_path / "rec.bin")
+    qvti.write_raw(out, rec)
+    assert np.array_equal(np.fromfile(out, np.float32).reshape(40, 9), rec)
