@@ -157,6 +157,26 @@ int qvti_set_autotune(qvti_ctx *c, int mode);
  * (0 for row kernels), in the class order of qvti_kernel_times; split may be
  * NULL.  Returns the number of classes. */
 int qvti_tile_choice(qvti_ctx *c, int max_classes, int *pencils, int *split);
+/* Opt-in k-space time-stepping correction and wavenumber filter of the
+ * stress operators.  Both are off by default (the reference scheme); the
+ * reference has them switched off: sinc2nd = 1 at cuviscoelas_3D.cu:183-190,
+ * the filter array commented out at Myfunctions.h:69 and its parameters
+ * unused at viscoelas_3D.cpp:111-112.
+ *  vref > 0: every spectral operator of the stress update (A1 = |k|^-1,
+ *    A2 = 1, A3 = |k|) is multiplied by sinc^2(vref |k| dt / 2),
+ *    sinc(x) = sin(x)/x, the reference's disabled sinc2nd with
+ *    velp_max = vref.  Waves of speed vref then propagate without time
+ *    dispersion, and a homogeneous medium with speeds <= vref is stable at
+ *    any dt.
+ *  fcut > 0: the same operators are multiplied by a radial Tukey taper of
+ *    |k|: 1 up to (1 - taper_ratio) kc, a half cosine down to 0 at kc, 0
+ *    beyond, with kc = 2 pi fcut / vmax (fcut in Hz, vmax in m/s,
+ *    taper_ratio in [0, 1]).
+ *  vref = 0 and fcut = 0: both off.
+ * Takes effect at the next qvti_begin; a run in progress must be begun
+ * again, as after qvti_set_snapshot.  The first call that switches it on allocates 6 more fields per
+ * shot; each step then runs three more complex 3-D inverse transforms. */
+int qvti_set_kspace(qvti_ctx *c, float vref, float fcut, float vmax, float taper_ratio);
 /* Blow-up probe interval in steps (0 disables; default 50) and limit on |v|. */
 int qvti_set_probe(qvti_ctx *c, int every, float limit);
 /* Device bytes this context holds. */

@@ -29,6 +29,8 @@ class QoConfig(ctypes.Structure):
         ("rz", ctypes.POINTER(ctypes.c_int)),
         ("vti", ctypes.c_int), ("snap_every", ctypes.c_int), ("snap_field", ctypes.c_int),
         ("nthreads", ctypes.c_int),
+        ("ks_vref", ctypes.c_float), ("ks_fcut", ctypes.c_float), ("ks_vmax", ctypes.c_float),
+        ("ks_taper", ctypes.c_float),
     ]
 
 
@@ -57,6 +59,8 @@ def lib():
             f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float] + [fp(dt, flags="C")] * 2
             f = getattr(_lib, "qo_run_" + sfx)
             f.argtypes = [ctypes.POINTER(QoConfig)] + [ctypes.c_void_p] * 13
+            f = getattr(_lib, "qo_kspace_factor_" + sfx)
+            f.argtypes = [ctypes.c_int, fp(dt, flags="C")] + [ctypes.c_float] * 5 + [fp(dt, flags="C")]
         _lib.qo_q_coeffs.argtypes = [ctypes.c_double] * 4 + [fp(np.float64, flags="C")]
     return _lib
 
@@ -104,16 +108,29 @@ def q_coeffs(rho, v, gamma, w0):
     return out
 
 
+def kspace_factor(kabs, dt, vref=0.0, fcut=0.0, vmax=0.0, taper=0.0, prec="f64"):
+    """The oracle's k-space factor F(|k|) of the stress operators."""
+    dt_ = np.float64 if prec == "f64" else np.float32
+    k = np.ascontiguousarray(kabs, dt_).ravel()
+    out = np.zeros_like(k)
+    rc = getattr(lib(), "qo_kspace_factor_" + prec)(len(k), k, dt, vref, fcut, vmax, taper, out)
+    if rc != 0:
+        raise ValueError("oracle rejected the k-space parameters")
+    return out.reshape(np.shape(kabs))
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
-def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_field=0):
+def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_field=0, kspace=None):
     """Run the oracle on one shot.
 
     shot: dict(nx, ny, nz, pml, nt, dx, dy, dz, dt, f0, w0, src_kind, sx, sy, sz,
                rx, ry, rz (int arrays), rik (float32 [nt]))
     model: dict(vp, gp, vs, gs, rho[, eps, delta]) float32 arrays of shape (nx, ny, nz)
+    kspace: optional dict(vref, fcut, vmax, taper) of the opt-in k-space
+    correction / wavenumber filter (absent: the reference scheme).
     Returns dict with records [3][nt][nrec] and optionally state [9][nx][ny][nz]
     and snapshots.
     """
@@ -130,6 +147,11 @@ def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_fie
                    ry.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                    rz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                    vti, snap_every, snap_field, nthreads)
+    if kspace:
+        cfg.ks_vref = kspace.get("vref", 0.0)
+        cfg.ks_fcut = kspace.get("fcut", 0.0)
+        cfg.ks_vmax = kspace.get("vmax", 0.0)
+        cfg.ks_taper = kspace.get("taper", 0.0)
     f32 = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
     rik = f32(shot["rik"])
     m = {k: f32(model.get(k)) for k in ("vp", "gp", "vs", "gs", "rho", "eps", "delta")}

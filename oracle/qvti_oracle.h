@@ -47,6 +47,10 @@ typedef struct {
     int snap_every;        /* 0: off; else snapshot when it % snap_every == 0 */
     int snap_field;        /* QO_VX ... QO_PYZ */
     int nthreads;          /* OpenMP threads (<=0: library default) */
+    /* opt-in k-space factor of the stress operators (all 0: the reference
+     * scheme): sinc^2(ks_vref |k| dt / 2) and a radial Tukey taper with
+     * cut-off 2 pi ks_fcut / ks_vmax and taper ratio ks_taper */
+    float ks_vref, ks_fcut, ks_vmax, ks_taper;
 } qo_config;
 
 /* Full forward run.  Model arrays are FP32 over the total grid, index
@@ -84,6 +88,12 @@ int qo_pml_profile_f32(int n, int pml, float d, float f0, float *a, float *b);
 /* Q-model coefficients for one cell: out[5] = eta1, eta2, eta3, tau1, tau2
  * (cuviscoelas_3D.cu:195-205), FP64. */
 void qo_q_coeffs(double rho, double v, double gamma, double w0, double *out);
+
+/* The k-space factor F(|k|) for given |k| values (closed-form tests). */
+int qo_kspace_factor_f64(int n, const double *kabs, float dt, float vref, float fcut, float vmax, float taper,
+                         double *out);
+int qo_kspace_factor_f32(int n, const float *kabs, float dt, float vref, float fcut, float vmax, float taper,
+                         float *out);
 
 int qo_version(void);
 

@@ -428,6 +428,65 @@ static inline double FN(sbar)(const REAL *const *cs, long ip, long sy, long sx,
     return acc;
 }
 
+/* Opt-in k-space factor of the stress operators, F(|k|) = sinc^2(vh |k|) W(|k|)
+ * with vh = vref dt / 2 and sinc(x) = sin(x)/x: the reference's disabled
+ * sinc2nd = pow(sin(velp_max |k| dt/2) / (velp_max |k| dt/2), 2)
+ * (cuviscoelas_3D.cu:183-184), multiplied into A1, A2, A3 as at :188-190;
+ * W is a radial Tukey taper, 1 for |k| <= k0 = (1 - r) kc, half cosine
+ * (1 + cos(pi (|k| - k0) / (kc - k0))) / 2 down to 0 at kc = 2 pi fcut / vmax,
+ * 0 beyond (the filter declared but never built: Myfunctions.h:69,
+ * viscoelas_3D.cpp:111-112).  vref = 0 / fcut = 0 drop either part. */
+typedef struct { REAL vh, k0, kc, pw; int on; } FN(ks_t);
+
+static int FN(ks_init)(FN(ks_t) *k, float dt, float vref, float fcut, float vmax, float taper)
+{
+    memset(k, 0, sizeof(*k));
+    if (!(vref >= 0) || !(fcut >= 0)) return -1;
+    if (fcut > 0 && (!(vmax > 0) || !(taper >= 0 && taper <= 1))) return -1;
+    k->vh = (REAL)(0.5 * (double)vref * (double)dt);
+    if (fcut > 0) {
+        double kc = 2.0 * QO_PI * (double)fcut / (double)vmax, k0 = (1.0 - (double)taper) * kc;
+        k->kc = (REAL)kc;
+        k->k0 = (REAL)k0;
+        k->pw = kc > k0 ? (REAL)(M_PI / (kc - k0)) : (REAL)0;
+    }
+    k->on = vref > 0 || fcut > 0;
+    return 0;
+}
+
+static REAL FN(ks_factor)(const FN(ks_t) *k, REAL ka)
+{
+    REAL f = 1;
+    if (k->vh > 0) {
+        REAL x = k->vh * ka;
+#if QO_IS_F32
+        REAL sx = x > 0 ? sinf(x) / x : (REAL)1;
+#else
+        REAL sx = x > 0 ? sin(x) / x : (REAL)1;
+#endif
+        f = sx * sx;
+    }
+    if (k->kc > 0) {
+#if QO_IS_F32
+        if (ka > k->kc) f = 0;
+        else if (ka > k->k0) f *= (REAL)0.5 * ((REAL)1 + cosf(k->pw * (ka - k->k0)));
+#else
+        if (ka > k->kc) f = 0;
+        else if (ka > k->k0) f *= 0.5 * (1.0 + cos(k->pw * (ka - k->k0)));
+#endif
+    }
+    return f;
+}
+
+int FN(qo_kspace_factor)(int n, const REAL *kabs, float dt, float vref, float fcut, float vmax, float taper,
+                         REAL *out)
+{
+    FN(ks_t) k;
+    if (n < 0 || FN(ks_init)(&k, dt, vref, fcut, vmax, taper)) return -1;
+    for (int i = 0; i < n; i++) out[i] = FN(ks_factor)(&k, kabs[i]);
+    return 0;
+}
+
 int FN(qo_run)(const qo_config *c, const float *rik,
                const float *vp, const float *gp, const float *vs, const float *gs,
                const float *rho_in, const float *eps, const float *del,
@@ -438,6 +497,8 @@ int FN(qo_run)(const qo_config *c, const float *rik,
         return -1;
     if (!rik || !vp || !gp || !vs || !gs || !rho_in) return -1;
     if (c->vti && (!eps || !del)) return -1;
+    FN(ks_t) ks;
+    if (FN(ks_init)(&ks, c->dt, c->ks_vref, c->ks_fcut, c->ks_vmax, c->ks_taper)) return -1;
     if (c->sx < 0 || c->sx >= nx || c->sy < 0 || c->sy >= ny || c->sz < 0 || c->sz >= nz) return -1;
     for (int r = 0; r < c->nrec; r++)
         if (c->rx[r] < 0 || c->rx[r] >= nx || c->ry[r] < 0 || c->ry[r] >= ny || c->rz[r] < 0 || c->rz[r] >= nz)
@@ -463,6 +524,8 @@ int FN(qo_run)(const qo_config *c, const float *rik,
     FN(qo_axis_tables)(nz, c->dz, kzf, dpz, dmz);
     /* A1 = k^-1/2, A2 = 1, A3 = k^1/2 with k = |k|^2, k[0] = 1e-10 (:72-74, 188-190) */
     REAL *A1 = malloc(sizeof(REAL) * N), *A3 = malloc(sizeof(REAL) * N);
+    /* k-space mode only: A2 = F, and F multiplies A1 and A3 */
+    REAL *A2 = ks.on ? malloc(sizeof(REAL) * N) : NULL;
 #pragma omp parallel for schedule(static)
     for (long ip = 0; ip < N; ip++) {
         int ix = (int)(ip / sx), iy = (int)((ip / sy) % ny), iz = (int)(ip % nz);
@@ -475,6 +538,12 @@ int FN(qo_run)(const qo_config *c, const float *rik,
         A1[ip] = pow(k2, -0.5);
         A3[ip] = pow(k2, 0.5);
 #endif
+        if (A2) {
+            REAL F = FN(ks_factor)(&ks, A3[ip]);
+            A1[ip] *= F;
+            A3[ip] *= F;
+            A2[ip] = F;
+        }
     }
 
     /* -------- Q coefficients (cuviscoelas_3D.cu:195-205), FP64 then REAL.
@@ -623,7 +692,7 @@ int FN(qo_run)(const qo_config *c, const float *rik,
         for (int g = 0; g < 9; g++) {
             const REAL *S = spec[6 + sop[g].comp], *P = dv[sop[g].comp];
             for (int j = 1; j <= 5; j++) {
-                const REAL *A = (j == 1 || j == 4) ? A1 : ((j == 3) ? A3 : NULL);
+                const REAL *A = (j == 1 || j == 4) ? A1 : ((j == 3) ? A3 : A2);
                 int AorB = (j >= 4);
 #pragma omp parallel for schedule(static)
                 for (long ip = 0; ip < N; ip++) {
@@ -734,7 +803,7 @@ int FN(qo_run)(const qo_config *c, const float *rik,
     for (int f = 0; f < 45; f++) free(pv[f]);
     for (int f = 0; f < NPHI; f++) free(phi[f]);
     for (int j = 0; j < 5; j++) { free(cP[j]); free(cS[j]); free(cH[j]); free(cD[j]); }
-    free(work); free(A1); free(A3);
+    free(work); free(A1); free(A2); free(A3);
     free(kxf); free(kyf); free(kzf); free(dpx); free(dmx); free(dpy); free(dmy); free(dpz); free(dmz);
     free(ax); free(bx); free(ay); free(by); free(az); free(bz);
     return 0;

@@ -276,6 +276,10 @@ struct qvti_ctx {
     int jz[KC_COUNT] = {};
     bool autotune = true;
     int tuned = 0;  // shot count the tile widths were tuned for (0: not tuned)
+    // opt-in k-space correction / wavenumber filter (off: the reference scheme)
+    bool ks = false;
+    float ks_vref = 0.f, ks_fcut = 0.f, ks_vmax = 0.f, ks_taper = 0.f;
+    float* H2 = nullptr;  // [A2 v x,y,z][A2 dv x,y,z], allocated on first use
     long long bytes = 0;
     std::vector<void*> allocs;
 };
@@ -570,6 +574,28 @@ static int build_launch_config(qvti_ctx* c)
     xp.Q3[0] = Q3[0];
     xp.Q3[1] = nullptr;
     xp.Q3[2] = Q3[1];
+    // k-space mode: A2 F P_c -> Q2_c in bufB fields 6-11 (free once the
+    // x-pass has read P; the y-derivatives overwrite them after z_inv)
+    float2* Q2[3] = {nullptr, nullptr, nullptr};
+    float* H2v[3] = {nullptr, nullptr, nullptr};
+    float* H2dv[3] = {nullptr, nullptr, nullptr};
+    xp.ks_vh = xp.ks_k0 = xp.ks_kc = xp.ks_pw = 0.f;
+    if (c->ks) {
+        for (int k = 0; k < 3; ++k) {
+            Q2[k] = reinterpret_cast<float2*>(fld(c->bufB, 6 + 2 * k, c));
+            H2v[k] = fld(c->H2, k, c);
+            H2dv[k] = fld(c->H2, 3 + k, c);
+            xp.Q2[k] = Q2[k];
+        }
+        xp.ks_vh = (float)(0.5 * (double)c->ks_vref * (double)g.dt);
+        if (c->ks_fcut > 0.f) {
+            const double kc = 2.0 * kRefPi * (double)c->ks_fcut / (double)c->ks_vmax;
+            const double k0 = (1.0 - (double)c->ks_taper) * kc;
+            xp.ks_kc = (float)kc;
+            xp.ks_k0 = (float)k0;
+            xp.ks_pw = kc > k0 ? (float)(M_PI / (kc - k0)) : 0.f;
+        }
+    }
 
     CStridedArgs& yi = c->a_yi;
     yi = yf;
@@ -577,6 +603,10 @@ static int build_launch_config(qvti_ctx* c)
     for (int k = 0; k < 3; ++k) yi.f[k] = Q1[k];
     yi.f[3] = Q3[0];
     yi.f[4] = Q3[1];
+    if (c->ks) {
+        yi.nfields = 8;
+        for (int k = 0; k < 3; ++k) yi.f[5 + k] = Q2[k];
+    }
 
     ZRowsArgs& zi = c->a_zi;
     zi.nrows = c->nrows;
@@ -590,11 +620,18 @@ static int build_launch_config(qvti_ctx* c)
     zi.cin[2] = Q1[2]; zi.re_out[2] = H1v[2]; zi.im_out[2] = H1dv[2];   // A1 vz, A1 dvz
     zi.cin[3] = Q3[0]; zi.re_out[3] = H3v[0]; zi.im_out[3] = H3v[1];    // A3 vx, A3 vy
     zi.cin[4] = Q3[1]; zi.re_out[4] = H3v[2]; zi.im_out[4] = nullptr;   // A3 vz
+    if (c->ks) {
+        zi.nfields = 8;
+        zi.cin[5] = Q2[0]; zi.re_out[5] = H2v[0]; zi.im_out[5] = H2v[1];    // A2 vx, A2 vy
+        zi.cin[6] = Q2[1]; zi.re_out[6] = H2dv[0]; zi.im_out[6] = H2dv[1];  // A2 dvx, A2 dvy
+        zi.cin[7] = Q2[2]; zi.re_out[7] = H2v[2]; zi.im_out[7] = H2dv[2];   // A2 vz, A2 dvz
+    }
 
     // ---- stress x / y derivatives of the 15 family fields
     const float* fam[3][5];
     for (int k = 0; k < 3; ++k) {
         fam[k][0] = H1v[k]; fam[k][1] = vv[k]; fam[k][2] = H3v[k]; fam[k][3] = H1dv[k]; fam[k][4] = dvv[k];
+        if (c->ks) { fam[k][1] = H2v[k]; fam[k][4] = H2dv[k]; }
     }
     // x/y derivatives of the stress step: blocked groups of 15 fields per
     // shot in bufA (x) and bufB (y); job k writes field k of the group
@@ -640,6 +677,7 @@ static int build_launch_config(qvti_ctx* c)
     sz.g1 = c->coef;
     sz.g2 = g.vti ? c->coef + (long long)kG1 * N : nullptr;
     for (int k = 0; k < 6; ++k) sz.sig[k] = sg[k];
+    for (int k = 0; k < 3; ++k) sz.vel[k] = vv[k];
     for (int k = 0; k < 5; ++k) {
         sz.phx[k] = c->phis[0] + (long long)k * c->S * c->slab[0];
         sz.phy[k] = c->phis[1] + (long long)k * c->S * c->slab[1];
@@ -726,7 +764,7 @@ static int record_event(qvti_ctx* c, int cls)
 // Enqueue one time step (14 launches).  With timing on, each launch is
 // bracketed by an event pair.
 // Workgroups per row tile of the final inverse z pass (1: all fields in one
-// workgroup; 5: one field each).
+// workgroup; otherwise one field each).
 #ifndef QVTI_ZINV_SPLIT
 #define QVTI_ZINV_SPLIT 5
 #endif
@@ -774,7 +812,8 @@ static void launch_class(qvti_ctx* c, int cls)
     case KC_XPASS:
         dispatch_nw(nx, c->pw[cls], [&](auto g) {
             using G = decltype(g);
-            hipLaunchKernelGGL(k2_xpass<G>, dim3(c->g_xp.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_xp);
+            if (c->ks) hipLaunchKernelGGL((k2_xpass<G, true>), dim3(c->g_xp.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_xp);
+            else hipLaunchKernelGGL(k2_xpass<G>, dim3(c->g_xp.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_xp);
         });
         break;
     case KC_Y_INV:
@@ -785,7 +824,8 @@ static void launch_class(qvti_ctx* c, int cls)
         break;
     case KC_Z_INV:
         dispatch_n(nz, [&](auto g) {
-            hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3(c->g_zr.x, S, QVTI_ZINV_SPLIT), blk, 0, st, c->a_zi);
+            const unsigned zs = QVTI_ZINV_SPLIT == 1 ? 1u : (unsigned)c->a_zi.nfields;
+            hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3(c->g_zr.x, S, zs), blk, 0, st, c->a_zi);
         });
         break;
     case KC_STR_X:
@@ -847,11 +887,11 @@ static void apply_tiles(qvti_ctx* c)
 }
 
 // Fields (jobs) one launch of a strided kernel class processes.
-static int class_fields(int cls)
+static int class_fields(const qvti_ctx* c, int cls)
 {
     switch (cls) {
     case KC_VEL_X: case KC_VEL_Y: case KC_Y_FWD: case KC_XPASS: return 3;
-    case KC_Y_INV: return 5;
+    case KC_Y_INV: return c->a_yi.nfields;  // 5, or 8 in k-space mode
     case KC_STR_X: case KC_STR_Y: return 15;
     default: return 1;
     }
@@ -874,7 +914,7 @@ static int autotune_tiles(qvti_ctx* c)
     int rc = QVTI_OK;
     for (int cls : classes) {
         const int n = (cls == KC_VEL_X || cls == KC_XPASS || cls == KC_STR_X) ? c->g.nx : c->g.ny;
-        const int nf = class_fields(cls);
+        const int nf = class_fields(c, cls);
         int best_pw = 1, best_jz = 1;
         float best = 1e30f;
         for (int pw : {1, 2, 4}) {
@@ -1245,9 +1285,50 @@ extern "C" int qvti_set_autotune(qvti_ctx* c, int mode)
     if (!c->autotune)
         for (int k = 0; k < KC_COUNT; ++k) {
             c->pw[k] = w;  // the widest available <= w is used
-            c->jz[k] = split ? class_fields(k) : 1;
+            c->jz[k] = split ? class_fields(c, k) : 1;
         }
     apply_tiles(c);
+    return QVTI_OK;
+}
+
+extern "C" int qvti_set_kspace(qvti_ctx* c, float vref, float fcut, float vmax, float taper_ratio)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    if (!(vref >= 0.f) || !std::isfinite(vref)) return qvti_fail(QVTI_E_ARG, "vref %g must be >= 0", vref);
+    if (!(fcut >= 0.f) || !std::isfinite(fcut)) return qvti_fail(QVTI_E_ARG, "fcut %g must be >= 0", fcut);
+    if (fcut > 0.f) {
+        if (!(vmax > 0.f) || !std::isfinite(vmax)) return qvti_fail(QVTI_E_ARG, "vmax %g must be > 0", vmax);
+        if (!(taper_ratio >= 0.f && taper_ratio <= 1.f))
+            return qvti_fail(QVTI_E_ARG, "taper_ratio %g not in [0, 1]", taper_ratio);
+    }
+    HIPCHK(hipSetDevice(c->device));
+    // the launch arguments change: let queued steps finish, and require a
+    // new qvti_begin (as qvti_set_snapshot does)
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->begun = 0;
+    const bool on = vref > 0.f || fcut > 0.f;
+    if (on && !c->H2) {
+        const int rc = dalloc(c, (void**)&c->H2, sizeof(float) * 6 * c->S * c->N);
+        if (rc) return rc;
+    }
+    c->ks = on;
+    c->ks_vref = vref;
+    c->ks_fcut = fcut;
+    c->ks_vmax = fcut > 0.f ? vmax : 0.f;
+    c->ks_taper = fcut > 0.f ? taper_ratio : 0.f;
+    // the launch arguments and the y-inverse field count change: rebuild
+    // them, keep the fixed tile widths a non-autotuned context was given
+    int pw[KC_COUNT], jz[KC_COUNT];
+    std::copy(c->pw, c->pw + KC_COUNT, pw);
+    std::copy(c->jz, c->jz + KC_COUNT, jz);
+    const int rc = build_launch_config(c);
+    if (rc) return rc;
+    if (!c->autotune) {
+        std::copy(pw, pw + KC_COUNT, c->pw);
+        for (int k = 0; k < KC_COUNT; ++k) c->jz[k] = jz[k] > 1 ? class_fields(c, k) : 1;
+        apply_tiles(c);
+    }
+    c->tuned = 0;
     return QVTI_OK;
 }
 
@@ -1322,9 +1403,11 @@ extern "C" int qvti_kernel_bytes(qvti_ctx* c, int max_classes, double* bytes)
     b[KC_VEL_Y] = 9 * f + 3 * 2 * sy;
     b[KC_VEL_Z] = (3 + 3 + 3 + 3 + 3 + 6) * f + 3.0 * 4 * c->N + 3 * 2 * sz;  // sig, acc, v r/w, dv, P, buoy
     b[KC_Y_FWD] = 12 * f;
-    b[KC_XPASS] = 6 * f + 10 * f;
-    b[KC_Y_INV] = 20 * f;
-    b[KC_Z_INV] = 10 * f + 9 * f;
+    // k-space mode: three more complex spectra through the x-pass, y-inverse
+    // and z-inverse, six more real fields out
+    b[KC_XPASS] = 6 * f + (c->ks ? 16 : 10) * f;
+    b[KC_Y_INV] = (c->ks ? 32 : 20) * f;
+    b[KC_Z_INV] = c->ks ? 16 * f + 15 * f : 10 * f + 9 * f;
     b[KC_STR_X] = 30 * f;
     b[KC_STR_Y] = 30 * f;
     b[KC_STR_Z] = (15 + 30 + 12) * f + ncoef * 4.0 * c->N + 2 * (5 * sx + 5 * sy + 4 * sz);

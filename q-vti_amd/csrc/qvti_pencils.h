@@ -456,7 +456,7 @@ struct CStridedArgs {
     long long shot_stride;
     const float2* tw;
     int nfields;
-    float2* f[6];
+    float2* f[8];
 };
 
 template <class G, int DIR>
@@ -519,6 +519,15 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
 // from the folded 1-D |K| tables, evaluated in FP64, cuviscoelas_3D.cu:72,
 // 188-190; the DC symbol of |k|^-1 is 0 because every use is followed by a
 // derivative that vanishes there), inverse x-FFT, write Q1, Q3.
+//
+// KS (opt-in k-space correction and wavenumber filter): every stress
+// operator A_j, A2 = 1 included, is multiplied by
+//   F(|k|) = sinc^2(vh |k|) * W(|k|),  sinc(x) = sin(x)/x,  vh = vref dt / 2
+// (the disabled sinc2nd of cuviscoelas_3D.cu:183-190), with W the radial
+// Tukey taper: 1 for |k| <= k0, (1 + cos(pi (|k| - k0) / (kc - k0))) / 2 up
+// to kc, 0 beyond (the filter the reference declares but never builds,
+// Myfunctions.h:69, viscoelas_3D.cpp:111-112).  A2 F is no longer a 1-D
+// operator, so a third product F P / N is inverse-transformed into Q2.
 struct XPassArgs {
     int ny, nz;
     long long sx, sy;
@@ -531,9 +540,29 @@ struct XPassArgs {
     const float2* P[3];
     float2* Q1[3];
     float2* Q3[3];
+    float2* Q2[3];           // KS only
+    float ks_vh;             // vref dt / 2 (0: no sinc^2 factor)
+    float ks_k0, ks_kc;      // taper start and cut-off (kc = 0: no filter)
+    float ks_pw;             // pi / (kc - k0) (0 when k0 == kc)
 };
 
-template <class G>
+// F(|k|) of the KS mode, in FP32.
+__device__ __forceinline__ float kspace_factor(const XPassArgs& a, float k)
+{
+    float f = 1.f;
+    if (a.ks_vh > 0.f) {
+        const float x = a.ks_vh * k;
+        const float sx = x > 0.f ? sinf(x) / x : 1.f;
+        f = sx * sx;
+    }
+    if (a.ks_kc > 0.f) {
+        if (k > a.ks_kc) f = 0.f;
+        else if (k > a.ks_k0) f *= 0.5f * (1.f + cosf(a.ks_pw * (k - a.ks_k0)));
+    }
+    return f;
+}
+
+template <class G, bool KS = false>
 __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS_WAVES))) void k2_xpass(XPassArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
@@ -569,9 +598,11 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
         const int nops = a.Q3[comp] ? 2 : 1;  // |k| (op 1) only where its output is used
         int jl = jj;  // opaque copy: the |k| symbols are computed here, not hoisted and held
         asm volatile("" : "+v"(jl));
+        // ops: 0 A1 -> Q1, 1 A3 -> Q3 (where used), 2 A2 -> Q2 (KS only)
 #pragma unroll
-        for (int op = 0; op < 2; ++op) {
-            if (op >= nops) break;
+        for (int op = 0; op < (KS ? 3 : 2); ++op) {
+            if (!KS && op >= nops) break;
+            if (KS && op == 1 && nops < 2) continue;
             __syncthreads();
             if (act2) {
                 float2 c[G::N2];
@@ -589,7 +620,18 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
                     // (at k = 0 the estimate is inf and y is NaN: both
                     // symbols are 0 there, |k|^-1 by the convention above)
                     double s = 0.0;
-                    if (k2s > 0.0) s = op == 0 ? a.invN * y : a.invN * (k2s * y);
+                    if (KS) {
+                        // at k = 0: F = 1, so only A2 F is nonzero there
+                        if (k2s > 0.0) {
+                            const double ka = k2s * y;
+                            const double F = (double)kspace_factor(a, (float)ka);
+                            s = op == 0 ? a.invN * y * F : (op == 1 ? a.invN * ka * F : a.invN * F);
+                        } else if (op == 2) {
+                            s = a.invN;
+                        }
+                    } else if (k2s > 0.0) {
+                        s = op == 0 ? a.invN * y : a.invN * (k2s * y);
+                    }
                     const float fs = (float)s;
                     c[k2] = make_float2(b[k2].x * fs, b[k2].y * fs);
                 }
@@ -599,7 +641,7 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
             if (act1) {
                 float2 d[G::N1];
                 inv_lb<G>(d, pen, jj);
-                float2* const dst = (op == 0 ? a.Q1[comp] : a.Q3[comp]) + tbase;
+                float2* const dst = (op == 0 ? a.Q1[comp] : (op == 1 ? a.Q3[comp] : a.Q2[comp])) + tbase;
 #pragma unroll
                 for (int j1 = 0; j1 < G::N1; ++j1) (dst + (long long)(G::N2 * j1) * sxi)[voff] = d[j1];
             }
@@ -616,10 +658,10 @@ struct ZRowsArgs {
     long long shot_stride;
     const float2* tw;
     int mode, dir, nfields;
-    const float2* cin[6];
-    float2* cout[6];
-    float* re_out[6];
-    float* im_out[6];
+    const float2* cin[8];
+    float2* cout[8];
+    float* re_out[8];
+    float* im_out[8];
 };
 
 template <class G>
@@ -760,7 +802,10 @@ struct StressZArgs {
     const float *pax, *pbx, *paxh, *pbxh;
     const float *pay, *pby, *payh, *pbyh;
     const float *paz, *pbz, *pazh, *pbzh;
-    const float* fam[3][5];  // [component][A1 v, v, A3 v, A1 dv, dv]
+    // [component][A1 v, A2 v, A3 v, A1 dv, A2 dv]; A2 = 1 (the fields v and
+    // dv themselves) unless the k-space correction is on
+    const float* fam[3][5];
+    float* vel[3];  // vx vy vz: force sources are added here
     // x- and y-derivatives, blocked groups of 15 per shot:
     //   dxb: ex 0-4 (dx vx-family), gx 5-9 (dx vz-family), hx 10-14 (dx vy-family)
     //   dyb: ey 0-4 (dy vy-family), gy 5-9 (dy vz-family), hy 10-14 (dy vx-family)
@@ -999,9 +1044,9 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
             a.sig[5][fbase + o] = sg[5] + dt * (A[12] + fyz + A[13] + fzy);
         }
         if (row * n + iz == sip) {
-            // source after the stress update; this workgroup is the only one
-            // that reads this row of v (its z-derivative was staged above),
-            // so the write cannot race.
+            // source after the stress update; no other workgroup of this
+            // launch touches this row of v (at most this one read it, for its
+            // z-derivative, staged above), so the write cannot race.
             const float s = a.rik[it];
             const long long g = fbase + o;
             if (kind == 3) {
@@ -1009,8 +1054,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
                 a.sig[1][g] += s;
                 a.sig[2][g] += s;
             } else if (kind >= 0 && kind < 3) {
-                float* vc = const_cast<float*>(a.fam[kind][1]);
-                vc[g] += s;
+                a.vel[kind][g] += s;
             }
         }
     }

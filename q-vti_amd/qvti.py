@@ -27,7 +27,7 @@ EXPORTS = [
     "qvti_ricker", "qvti_gamma_from_q", "qvti_reference_receivers", "qvti_make_model",
     "qvti_server_create", "qvti_server_submit", "qvti_server_poll", "qvti_server_wait",
     "qvti_server_cancel", "qvti_server_clock", "qvti_server_destroy", "qvti_set_autotune", "qvti_tile_choice",
-    "qvti_pad_model", "qvti_load_model_raw", "qvti_write_raw",
+    "qvti_pad_model", "qvti_load_model_raw", "qvti_write_raw", "qvti_set_kspace",
 ]
 
 
@@ -99,6 +99,7 @@ def lib():
             "qvti_server_destroy": [vp], "qvti_set_autotune": [vp, i], "qvti_tile_choice": [vp, i, vp, vp],
             "qvti_pad_model": [vp, i, i, i, i, vp], "qvti_load_model_raw": [ctypes.c_char_p, i, i, i, i, vp],
             "qvti_write_raw": [ctypes.c_char_p, vp, ll],
+            "qvti_set_kspace": [vp, f, f, f, f],
         }
         for name, args in sig.items():
             if not hasattr(L, name):
@@ -289,6 +290,12 @@ class Engine:
         spl = np.zeros(32, np.int32)
         k = check(lib().qvti_tile_choice(self._c, 32, _p(out), _p(spl)))
         return {n: int(spl[j]) for j, n in enumerate(names[:k]) if out[j] > 0}
+
+    def set_kspace(self, vref=0.0, fcut=0.0, vmax=0.0, taper_ratio=0.0):
+        """Opt-in k-space correction sinc^2(vref |k| dt / 2) and radial Tukey
+        wavenumber filter (cut-off 2 pi fcut / vmax) on the stress operators;
+        all zero (the default) is the reference scheme."""
+        check(lib().qvti_set_kspace(self._c, vref, fcut, vmax, taper_ratio))
 
     def set_probe(self, every, limit=1e30):
         check(lib().qvti_set_probe(self._c, every, limit))

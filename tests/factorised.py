@@ -23,12 +23,19 @@ def d1(f,sym,axis):
 def pml(n,p,d,f0):
     a,b=O.pml_profile(n,p,d,f0,'f64'); ah=a.copy(); bh=b.copy()
     ah[:-1]=0.5*(a[:-1]+a[1:]); bh[:-1]=0.5*(b[:-1]+b[1:]); return a,b,ah,bh
-def run(shot, m, vti=False):
+def run(shot, m, vti=False, kspace=None):
     nx,ny,nz,P,nt=shot['nx'],shot['ny'],shot['nz'],shot['pml'],shot['nt']
     dt=np.float64(np.float32(shot['dt'])); N=nx*ny*nz
     kx,dpx,dmx=axis_syms(nx,shot['dx']); ky,dpy,dmy=axis_syms(ny,shot['dy']); kz,dpz,dmz=axis_syms(nz,shot['dz'])
     K2=kx[:,None,None]**2+ky[None,:,None]**2+kz[None,None,:]**2
     A1=np.where(K2>0,1/np.sqrt(np.where(K2>0,K2,1)),0); A3=np.sqrt(K2)
+    # opt-in k-space factor F(|k|) (the oracle's, pinned by closed forms):
+    # F multiplies A1 and A3, and A2 = F becomes a 3-D operator
+    F=None
+    if kspace:
+        F=O.kspace_factor(np.sqrt(K2), np.float32(shot['dt']), kspace.get('vref',0.0), kspace.get('fcut',0.0),
+                          kspace.get('vmax',0.0), kspace.get('taper',0.0))
+        A1=A1*F; A3=A3*F
     def q(rho,v,g):
         return np.array([O.q_coeffs(r,vv,gg,float(np.float32(shot['w0']))) for r,vv,gg in zip(rho.ravel(),v.ravel(),g.ravel())]).T.reshape(5,nx,ny,nz)
     rho=m['rho'].astype(np.float64)
@@ -71,7 +78,11 @@ def run(shot, m, vti=False):
         H1v=[np.fft.ifftn(A1*np.fft.fftn(v[c])).real for c in range(3)]
         H1w=[np.fft.ifftn(A1*np.fft.fftn(w[c])).real for c in range(3)]
         H3v=[np.fft.ifftn(A3*np.fft.fftn(v[c])).real for c in range(3)]
-        fam=[[H1v[c],v[c],H3v[c],H1w[c],w[c]] for c in range(3)]
+        if F is None: fam=[[H1v[c],v[c],H3v[c],H1w[c],w[c]] for c in range(3)]
+        else:
+            A2v=[np.fft.ifftn(F*np.fft.fftn(v[c])).real for c in range(3)]
+            A2w=[np.fft.ifftn(F*np.fft.fftn(w[c])).real for c in range(3)]
+            fam=[[H1v[c],A2v[c],H3v[c],H1w[c],A2w[c]] for c in range(3)]
         S=lambda C,fl,sym,ax: sum(C[j]*d1(fl[j],sym,ax) for j in range(5))
         c12=cH-2*cS; c13d=cD-2*cS
         s11x=S(cH,fam[0],dpx,0); s12x=S(c12,fam[0],dpx,0); s13x=S(c13d,fam[0],dpx,0)
