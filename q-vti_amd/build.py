@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqvti.so")
 
 HIP_SOURCES = ["qvti_engine.hip", "qvti_serve.hip"]
-CPP_SOURCES = ["qvti_host.cpp"]
+CPP_SOURCES = ["qvti_host.cpp", "qvti_error.cpp"]
 ARCH = os.environ.get("QVTI_ARCH", "gfx950")
 
 

@@ -22,20 +22,7 @@
 using namespace qvti;
 
 // ------------------------------------------------------------ errors -----
-static thread_local std::string g_err;
-
-int qvti_fail(int code, const char* fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-extern "C" const char* qvti_last_error(void) { return g_err.c_str(); }
+// qvti_fail / qvti_last_error live in qvti_error.cpp.
 
 #define HIPCHK(x)                                                                          \
     do {                                                                                   \
@@ -426,10 +413,10 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
     rc = rc ? rc : dalloc(c, (void**)&c->d_rec_off, sizeof(int) * (S + 1));
     if (!rc) rc = build_launch_config(c);
     if (rc) {
-        std::string keep = g_err;
+        std::string keep = qvti_last_error();
         release(c);
         delete c;
-        g_err = keep;
+        qvti_fail(rc, "%s", keep.c_str());
         return rc;
     }
     *out = c;
@@ -1098,7 +1085,7 @@ extern "C" int qvti_begin(qvti_ctx* c, int nt, const float* rik, const qvti_shot
         rc = capture_graph(c);
         if (rc) {
             c->use_graphs = false;  // fall back to direct launches of the same kernels
-            g_err.clear();
+            qvti_clear_error();
         }
     }
     return QVTI_OK;

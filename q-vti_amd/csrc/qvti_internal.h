@@ -3,3 +3,5 @@
 
 // Record a formatted message as this thread's last error and return `code`.
 int qvti_fail(int code, const char* fmt, ...);
+// Clear this thread's last error.
+void qvti_clear_error();
