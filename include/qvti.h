@@ -135,6 +135,18 @@ int qvti_sync(qvti_ctx *c);              /* wait; returns QVTI_E_UNSTABLE if the
 int qvti_finish(qvti_ctx *c, float *rec_vx, float *rec_vy, float *rec_vz);
 int qvti_get_state(qvti_ctx *c, int shot, int field, float *out); /* [nx][ny][nz] */
 int qvti_steps_done(qvti_ctx *c);
+/* Checkpoint / resume of a run in progress (every shot of the context).  The
+ * reference allocates checkpoint bookkeeping it never uses (check_steps,
+ * N_cp, t_cp at viscoelas_3D.cpp:223-229, 457).  The image holds the state
+ * that carries from step to step - velocities, stresses, C-PML memory - plus
+ * the step count, the records gathered so far and the snapshots taken so far.
+ * qvti_checkpoint waits for the queued steps.  qvti_restore loads an image
+ * into a context of the same grid that was given the same qvti_begin (nt,
+ * wavelet, shots, snapshot settings) and has not stepped yet; the run then
+ * continues from the saved step, bit-identical to an uninterrupted run. */
+long long qvti_checkpoint_bytes(qvti_ctx *c);
+int qvti_checkpoint(qvti_ctx *c, void *buf, long long capacity);
+int qvti_restore(qvti_ctx *c, const void *buf, long long size);
 /* Per-kernel-class HIP event timing of the launches enqueued while enabled.
  * qvti_kernel_times returns, per class, launches and total milliseconds. */
 int qvti_enable_kernel_timing(qvti_ctx *c, int on);

@@ -1172,6 +1172,117 @@ extern "C" int qvti_finish(qvti_ctx* c, float* rec_vx, float* rec_vy, float* rec
     return rc;
 }
 
+// ------------------------------------------------------ checkpoint ------
+// Image of a run in progress: header, then for shots [0, nshots) the fields
+// that carry from one step to the next (v, sig, the C-PML slabs), the records
+// gathered so far and the snapshots taken so far.  dv, the spectral buffers
+// and the derivative buffers are rebuilt inside every step.
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int version, nx, ny, nz, pml, vti, nshots, nt, steps_done, total_rec, snap_every, snap_field, probe;
+    long long nfloats;
+};
+const char kCkptMagic[8] = {'Q', 'V', 'T', 'I', 'C', 'K', 'P', '1'};
+
+// (device pointer, floats) regions of the image, in order
+std::vector<std::pair<float*, long long>> ckpt_regions(qvti_ctx* c)
+{
+    std::vector<std::pair<float*, long long>> r;
+    const long long n = (long long)c->nshots * c->N;
+    for (int f = 0; f < 3; ++f) r.push_back({fld(c->v, f, c), n});
+    for (int f = 0; f < 6; ++f) r.push_back({fld(c->sig, f, c), n});
+    for (int a = 0; a < 3; ++a) {
+        const long long sl = (long long)c->nshots * c->slab[a], stride = (long long)c->S * c->slab[a];
+        for (int f = 0; f < 3; ++f) r.push_back({c->phiv[a] + f * stride, sl});
+        for (int f = 0; f < c->nphis[a]; ++f) r.push_back({c->phis[a] + f * stride, sl});
+    }
+    if (c->total_rec > 0) r.push_back({c->d_rec, 3LL * c->total_rec * c->nt});
+    if (c->snap_every > 0 && c->d_snap && c->steps_done > 0) {
+        const qvti_grid& g = c->g;
+        const long long W = (long long)(g.nx - 2 * g.pml) * (g.ny - 2 * g.pml) * (g.nz - 2 * g.pml);
+        const long long taken = (c->steps_done - 1) / c->snap_every + 1;
+        r.push_back({c->d_snap, std::min(W * taken, c->d_snap_cap)});
+    }
+    return r;
+}
+
+CkptHeader ckpt_header(qvti_ctx* c, int probe)
+{
+    CkptHeader h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, kCkptMagic, 8);
+    h.version = 1;
+    h.nx = c->g.nx; h.ny = c->g.ny; h.nz = c->g.nz; h.pml = c->g.pml; h.vti = c->g.vti;
+    h.nshots = c->nshots; h.nt = c->nt; h.steps_done = c->steps_done; h.total_rec = c->total_rec;
+    h.snap_every = c->snap_every; h.snap_field = c->snap_field; h.probe = probe;
+    for (auto& p : ckpt_regions(c)) h.nfloats += p.second;
+    return h;
+}
+}  // namespace
+
+extern "C" long long qvti_checkpoint_bytes(qvti_ctx* c)
+{
+    if (!c || !c->begun) return qvti_fail(QVTI_E_STATE, "no run in progress");
+    return (long long)sizeof(CkptHeader) + 4LL * ckpt_header(c, 0).nfloats;
+}
+
+extern "C" int qvti_checkpoint(qvti_ctx* c, void* buf, long long capacity)
+{
+    if (!c || !buf) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    if (!c->begun) return qvti_fail(QVTI_E_STATE, "no run in progress");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int probe = 0;
+    HIPCHK(hipMemcpy(&probe, c->d_probe, sizeof(int), hipMemcpyDeviceToHost));
+    const CkptHeader h = ckpt_header(c, probe);
+    const long long need = (long long)sizeof h + 4LL * h.nfloats;
+    if (capacity < need) return qvti_fail(QVTI_E_ARG, "checkpoint needs %lld bytes, buffer holds %lld", need, capacity);
+    char* p = static_cast<char*>(buf);
+    memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    for (auto& r : ckpt_regions(c)) {
+        HIPCHK(hipMemcpy(p, r.first, 4 * r.second, hipMemcpyDeviceToHost));
+        p += 4 * r.second;
+    }
+    return QVTI_OK;
+}
+
+extern "C" int qvti_restore(qvti_ctx* c, const void* buf, long long size)
+{
+    if (!c || !buf) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    if (!c->begun) return qvti_fail(QVTI_E_STATE, "qvti_begin the run to restore first");
+    if (c->steps_done != 0) return qvti_fail(QVTI_E_STATE, "restore before the first qvti_step");
+    CkptHeader h;
+    if (size < (long long)sizeof h) return qvti_fail(QVTI_E_ARG, "checkpoint image too small");
+    memcpy(&h, buf, sizeof h);
+    if (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1) return qvti_fail(QVTI_E_ARG, "not a checkpoint image");
+    if (h.nx != c->g.nx || h.ny != c->g.ny || h.nz != c->g.nz || h.pml != c->g.pml || h.vti != c->g.vti ||
+        h.nshots != c->nshots || h.nt != c->nt || h.total_rec != c->total_rec || h.snap_every != c->snap_every ||
+        h.snap_field != c->snap_field)
+        return qvti_fail(QVTI_E_ARG, "checkpoint of a different run (grid, shots, nt, receivers or snapshots differ)");
+    if (h.steps_done < 0 || h.steps_done > h.nt) return qvti_fail(QVTI_E_ARG, "checkpoint step %d out of range", h.steps_done);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->steps_done = h.steps_done;  // sizes the snapshot region
+    const auto regions = ckpt_regions(c);
+    long long nf = 0;
+    for (auto& r : regions) nf += r.second;
+    if (nf != h.nfloats || size < (long long)sizeof h + 4 * nf) {
+        c->steps_done = 0;
+        return qvti_fail(QVTI_E_ARG, "checkpoint image size does not match the run");
+    }
+    const char* p = static_cast<const char*>(buf) + sizeof h;
+    for (auto& r : regions) {
+        HIPCHK(hipMemcpy(r.first, p, 4 * r.second, hipMemcpyHostToDevice));
+        p += 4 * r.second;
+    }
+    const int it = h.steps_done - 1;  // k_tick advances it at the start of each step
+    HIPCHK(hipMemcpy(c->d_it, &it, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_probe, &h.probe, sizeof(int), hipMemcpyHostToDevice));
+    return QVTI_OK;
+}
+
 extern "C" int qvti_get_state(qvti_ctx* c, int shot, int field, float* out)
 {
     if (!c || !out) return qvti_fail(QVTI_E_ARG, "NULL argument");

@@ -28,6 +28,7 @@ EXPORTS = [
     "qvti_server_create", "qvti_server_submit", "qvti_server_poll", "qvti_server_wait",
     "qvti_server_cancel", "qvti_server_clock", "qvti_server_destroy", "qvti_set_autotune", "qvti_tile_choice",
     "qvti_pad_model", "qvti_load_model_raw", "qvti_write_raw", "qvti_set_kspace",
+    "qvti_checkpoint_bytes", "qvti_checkpoint", "qvti_restore",
 ]
 
 
@@ -100,6 +101,7 @@ def lib():
             "qvti_pad_model": [vp, i, i, i, i, vp], "qvti_load_model_raw": [ctypes.c_char_p, i, i, i, i, vp],
             "qvti_write_raw": [ctypes.c_char_p, vp, ll],
             "qvti_set_kspace": [vp, f, f, f, f],
+            "qvti_checkpoint_bytes": [vp], "qvti_checkpoint": [vp, vp, ll], "qvti_restore": [vp, vp, ll],
         }
         for name, args in sig.items():
             if not hasattr(L, name):
@@ -112,6 +114,8 @@ def lib():
         L.qvti_device_bytes.restype = ll
         L.qvti_estimate_bytes.argtypes = [ctypes.POINTER(Grid), i]
         L.qvti_estimate_bytes.restype = ll
+        if hasattr(L, "qvti_checkpoint_bytes"):
+            L.qvti_checkpoint_bytes.restype = ll
         if hasattr(L, "qvti_server_clock"):
             L.qvti_server_clock.argtypes = [vp]
             L.qvti_server_clock.restype = d
@@ -296,6 +300,23 @@ class Engine:
         wavenumber filter (cut-off 2 pi fcut / vmax) on the stress operators;
         all zero (the default) is the reference scheme."""
         check(lib().qvti_set_kspace(self._c, vref, fcut, vmax, taper_ratio))
+
+    def steps_done(self):
+        return lib().qvti_steps_done(self._c)
+
+    def checkpoint(self):
+        """Image (uint8 array) of the run in progress, after the queued steps."""
+        n = lib().qvti_checkpoint_bytes(self._c)
+        if n < 0:
+            check(int(n))
+        img = np.zeros(n, np.uint8)
+        check(lib().qvti_checkpoint(self._c, _p(img), n))
+        return img
+
+    def restore(self, img):
+        """Continue a run from an image; call right after the matching begin."""
+        img = np.ascontiguousarray(img, np.uint8)
+        check(lib().qvti_restore(self._c, _p(img), img.size))
 
     def set_probe(self, every, limit=1e30):
         check(lib().qvti_set_probe(self._c, every, limit))

@@ -120,3 +120,42 @@ def test_exact_linearity_in_the_source_at_full_size(workload):
     for x, y in zip(a, b):
         assert np.array_equal(2 * x, y)
     assert np.array_equal(single, a[1])
+
+
+def test_checkpoint_restore_continues_bit_identically():
+    """A run checkpointed after 17 of 40 steps and restored into a fresh
+    context (records so far, snapshots, C-PML memory included) finishes with
+    the same bits as the uninterrupted run, for a 2-shot batch; an image of a
+    different run is refused."""
+    shot, model = small_case(24, 20, 32, 4, 40, O.SRC_EXPLOSIVE, True)
+    shots = [shot_of(shot), qvti.make_shot(O.SRC_FZ, (8, 9, 12), shot["rx"], shot["ry"], shot["rz"])]
+    nt, k = shot["nt"], 17
+
+    def run(split):
+        with engine_for(shot, model, max_shots=2) as e:
+            snaps = e.set_snapshot(4, 6, nt)
+            e.begin(nt, shot["rik"], shots)
+            if split:
+                e.step(k)
+                img = e.checkpoint()
+                e.step(5)  # work after the checkpoint is discarded by the restore below
+                with engine_for(shot, model, max_shots=2) as f:
+                    snaps = f.set_snapshot(4, 6, nt)
+                    f.begin(nt, shot["rik"], shots)
+                    f.restore(img)
+                    assert f.steps_done() == k
+                    f.step(nt - k)
+                    return f.finish(), snaps.copy(), img
+            e.step(nt)
+            return e.finish(), snaps.copy(), None
+
+    (a, sa, _), (b, sb, img) = run(False), run(True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert np.array_equal(sa, sb)
+    with engine_for(shot, model, max_shots=2) as e:
+        e.set_snapshot(4, 6, nt)
+        e.begin(nt - 1, shot["rik"], shots)
+        with pytest.raises(qvti.QvtiError) as err:
+            e.restore(img)
+        assert err.value.code == qvti.E_ARG
