@@ -598,6 +598,16 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
         const int nops = a.Q3[comp] ? 2 : 1;  // |k| (op 1) only where its output is used
         int jl = jj;  // opaque copy: the |k| symbols are computed here, not hoisted and held
         asm volatile("" : "+v"(jl));
+        if (KS && act2) {
+            // F multiplies every operator: apply it to the spectrum once (one
+            // more FP32 rounding than folding it into each symbol)
+#pragma unroll
+            for (int k2 = 0; k2 < G::N2; ++k2) {
+                const float kx = a.kfx[jl + G::N1 * k2];
+                const float F = kspace_factor(a, sqrtf((float)(kx * kx + kyz)));
+                b[k2] = make_float2(b[k2].x * F, b[k2].y * F);
+            }
+        }
         // ops: 0 A1 -> Q1, 1 A3 -> Q3 (where used), 2 A2 -> Q2 (KS only)
 #pragma unroll
         for (int op = 0; op < (KS ? 3 : 2); ++op) {
@@ -620,18 +630,8 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
                     // (at k = 0 the estimate is inf and y is NaN: both
                     // symbols are 0 there, |k|^-1 by the convention above)
                     double s = 0.0;
-                    if (KS) {
-                        // at k = 0: F = 1, so only A2 F is nonzero there
-                        if (k2s > 0.0) {
-                            const double ka = k2s * y;
-                            const double F = (double)kspace_factor(a, (float)ka);
-                            s = op == 0 ? a.invN * y * F : (op == 1 ? a.invN * ka * F : a.invN * F);
-                        } else if (op == 2) {
-                            s = a.invN;
-                        }
-                    } else if (k2s > 0.0) {
-                        s = op == 0 ? a.invN * y : a.invN * (k2s * y);
-                    }
+                    if (KS && op == 2) s = a.invN;  // A2 = 1 (F already applied)
+                    else if (k2s > 0.0) s = op == 0 ? a.invN * y : a.invN * (k2s * y);
                     const float fs = (float)s;
                     c[k2] = make_float2(b[k2].x * fs, b[k2].y * fs);
                 }
