@@ -748,10 +748,8 @@ static int record_event(qvti_ctx* c, int cls)
     return k;
 }
 
-// Enqueue one time step (14 launches).  With timing on, each launch is
-// bracketed by an event pair.
-// Workgroups per row tile of the final inverse z pass (1: all fields in one
-// workgroup; otherwise one field each).
+// Workgroups per row tile of the final inverse z pass: 1 puts all fields in
+// one workgroup, any other value one field per workgroup.
 #ifndef QVTI_ZINV_SPLIT
 #define QVTI_ZINV_SPLIT 5
 #endif

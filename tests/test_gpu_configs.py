@@ -37,12 +37,14 @@ def test_config1_iso64_500_steps_vs_golden():
     assert e32 < 1e-5 and e64 < 1e-5
 
 
-def _run_window(g, m, shot_kind, src, rx, ry, rz, nt):
+def _run_window(g, m, shot_kind, src, rx, ry, rz, nt, kspace=None):
     """GPU records and FP32 / FP64 oracle records of the first nt steps."""
     rik = qvti.ricker(nt, g["dt"], g["f0"])
     with qvti.Engine(g["nx"], g["ny"], g["nz"], g["pml"], g["dx"], g["dy"], g["dz"], g["dt"], g["f0"], g["w0"],
                      vti=g["vti"]) as e:
         e.set_model_dict(m)
+        if kspace:
+            e.set_kspace(kspace["vref"])
         rec = e.forward(nt, rik, qvti.make_shot(shot_kind, src, rx, ry, rz))
     shot = dict(nx=g["nx"], ny=g["ny"], nz=g["nz"], pml=g["pml"], nt=nt, dx=g["dx"], dy=g["dy"], dz=g["dz"],
                 dt=g["dt"], f0=g["f0"], w0=g["w0"], src_kind=shot_kind, sx=src[0], sy=src[1], sz=src[2],
@@ -50,8 +52,8 @@ def _run_window(g, m, shot_kind, src, rx, ry, rz, nt):
     model = {k: m[k] for k in ("vp", "gp", "vs", "gs", "rho")}
     if g["vti"]:
         model["eps"], model["delta"] = m["eps"], m["delta"]
-    r32 = O.run(shot, model, "f32")["records"]
-    r64 = O.run(shot, model, "f64")["records"]
+    r32 = O.run(shot, model, "f32", kspace=kspace)["records"]
+    r64 = O.run(shot, model, "f64", kspace=kspace)["records"]
     return rec, r32, r64
 
 
@@ -97,4 +99,22 @@ def test_config3_full_size_window(workload, nt):
     rec, r32, r64 = _run_window(g, m, qvti.SRC_FZ, src, rx, np.full(len(rx), src[1]), np.full(len(rx), src[2]), nt)
     e32, e64 = rel_l2(rec, r32), rel_l2(rec, r64)
     print("%s %d steps: vs FP32 oracle %.3e, vs FP64 oracle %.3e" % (workload, nt, e32, e64))
+    assert e32 < 1e-5 and e64 < 1e-5
+
+
+def test_config3_kspace_window_at_the_specified_dt():
+    """The 256^3 configuration at its specified 1 ms step with the opt-in
+    k-space correction (vref = 1.25 x the fastest horizontal qP speed, which
+    covers the Q dispersion of Q >= 20): 8 full-size steps against the FP32 /
+    FP64 oracle in the same mode, on a receiver line through the source."""
+    import bench
+
+    g, m, _ = bench.make_workload("vti256")
+    vh = float((m["vp"].astype(np.float64) * np.sqrt(1 + 2 * m["eps"].astype(np.float64))).max())
+    src = (g["nx"] // 2, g["ny"] // 2, g["pml"] + 1)
+    rx = np.arange(src[0] - 8, src[0] + 9)
+    rec, r32, r64 = _run_window(g, m, qvti.SRC_FZ, src, rx, np.full(len(rx), src[1]), np.full(len(rx), src[2]), 8,
+                                kspace=dict(vref=float(np.float32(1.25 * vh))))
+    e32, e64 = rel_l2(rec, r32), rel_l2(rec, r64)
+    print("vti256 k-space 8 steps: vs FP32 oracle %.3e, vs FP64 oracle %.3e" % (e32, e64))
     assert e32 < 1e-5 and e64 < 1e-5
