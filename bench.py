@@ -92,21 +92,35 @@ def run_plain(name, steps, warmup, kspace_vref=0.0):
     e.begin(nt, rik, [qvti.make_shot(sh["kind"], sh["src"], sh["rx"], sh["ry"], sh["rz"])])
     begin_s = time.perf_counter() - t0
     tiles = {"pencils": e.tile_choice(), "split": e.tile_split()}
+    blow_up = []
+
+    def sync():
+        # qvti_sync waits for the stream before it checks the blow-up probe,
+        # so the timing stands when the probe fires: a workload whose dt is
+        # beyond the scheme's stability limit still does the same work per step
+        try:
+            e.sync()
+        except qvti.QvtiError as ex:
+            if ex.code != qvti.E_UNSTABLE:
+                raise
+            if not blow_up:
+                blow_up.append(e.steps_done())
+
     e.step(warmup)
-    e.sync()
+    sync()
     t0 = time.perf_counter()
     e.step(steps)
-    e.sync()
+    sync()
     wall = time.perf_counter() - t0
     e.enable_kernel_timing(steps)
-    e.sync()
+    sync()
     t0 = time.perf_counter()
     e.step(steps)
-    e.sync()
+    sync()
     wall_timed = time.perf_counter() - t0
     kt = e.kernel_times()
     e.close()
-    return g, wall, wall_timed, kt, setup_s, begin_s, tiles
+    return g, wall, wall_timed, kt, setup_s, begin_s, tiles, (blow_up[0] if blow_up else None)
 
 
 def measured_traffic(name, kernel):
@@ -360,7 +374,7 @@ def main():
         print(json.dumps(run_served(args.served)), flush=True)
         return
     name = args.workload
-    g, wall, wall_timed, kt, setup_s, begin_s, tiles = run_plain(name, args.steps, args.warmup, args.kspace)
+    g, wall, wall_timed, kt, setup_s, begin_s, tiles, blow_up = run_plain(name, args.steps, args.warmup, args.kspace)
     N = g["nx"] * g["ny"] * g["nz"]
     gcs = N * args.steps / wall / 1e9
     balg = b_alg(g["nx"], g["ny"], g["nz"], g["pml"], g["vti"])
@@ -406,6 +420,9 @@ def main():
     }
     if args.kspace > 0:
         out["config"]["kspace_vref"] = args.kspace
+    if blow_up is not None:
+        out["stability"] = ("the blow-up probe fired by step %d: dt %.0e s is beyond the scheme's stability limit "
+                            "for this model (the work per step is unchanged)" % (blow_up, g["dt"]))
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = run_cpu_baseline(name)

@@ -459,6 +459,8 @@ static int build_launch_config(qvti_ctx* c)
         a.sx = sx; a.sy = sy;
         a.axis = axis;
         a.ztiles = (nz + 2 * P - 1) / (2 * P);
+        a.nplanes = axis == 0 ? ny : nx;
+        a.xcd = 1;
         a.dt = g.dt;
         a.shot_stride = N;
         a.slab_stride = c->slab[axis];
@@ -542,6 +544,8 @@ static int build_launch_config(qvti_ctx* c)
     yf.nz = nz; yf.sx = sx; yf.sy = sy;
     yf.axis = 1;
     yf.ztiles = (nz + Py - 1) / Py;
+    yf.nplanes = nx;
+    yf.xcd = 1;
     yf.shot_stride = N;
     yf.tw = Y.tw;
     yf.nfields = 3;
@@ -550,6 +554,7 @@ static int build_launch_config(qvti_ctx* c)
     XPassArgs& xp = c->a_xp;
     xp.ny = ny; xp.nz = nz; xp.sx = sx; xp.sy = sy;
     xp.ztiles = (nz + Px - 1) / Px;
+    xp.xcd = 1;
     xp.shot_stride = N;
     xp.tw = X.tw;
     xp.kfx = X.kf; xp.kfy = Y.kf; xp.kfz = Z.kf;
@@ -862,13 +867,15 @@ static void apply_tiles(qvti_ctx* c)
         const int w = tile_w(n, c->pw[cls]).P * cols;  // z-columns per tile
         return (nz + w - 1) / w;
     };
-    c->a_vx.ztiles = zt(nx, KC_VEL_X, 2); c->g_vx = dim3(ny * c->a_vx.ztiles);
-    c->a_sx.ztiles = zt(nx, KC_STR_X, 2); c->g_sx = dim3(ny * c->a_sx.ztiles);
-    c->a_vy.ztiles = zt(ny, KC_VEL_Y, 2); c->g_vy = dim3(nx * c->a_vy.ztiles);
-    c->a_sy.ztiles = zt(ny, KC_STR_Y, 2); c->g_sy = dim3(nx * c->a_sy.ztiles);
-    c->a_yf.ztiles = zt(ny, KC_Y_FWD, 1); c->g_cy = dim3(nx * c->a_yf.ztiles);
-    c->a_yi.ztiles = zt(ny, KC_Y_INV, 1); c->g_yi = dim3(nx * c->a_yi.ztiles);
-    c->a_xp.ztiles = zt(nx, KC_XPASS, 1); c->g_xp = dim3(ny * c->a_xp.ztiles);
+    // XCD-aware grid of nplanes planes of ztiles tiles (plane_tile() with xcd = 1)
+    auto grid = [&](int nplanes, int ztiles) { return dim3((unsigned)((nplanes + 7) / 8) * 8u * ztiles); };
+    c->a_vx.ztiles = zt(nx, KC_VEL_X, 2); c->g_vx = grid(ny, c->a_vx.ztiles);
+    c->a_sx.ztiles = zt(nx, KC_STR_X, 2); c->g_sx = grid(ny, c->a_sx.ztiles);
+    c->a_vy.ztiles = zt(ny, KC_VEL_Y, 2); c->g_vy = grid(nx, c->a_vy.ztiles);
+    c->a_sy.ztiles = zt(ny, KC_STR_Y, 2); c->g_sy = grid(nx, c->a_sy.ztiles);
+    c->a_yf.ztiles = zt(ny, KC_Y_FWD, 1); c->g_cy = grid(nx, c->a_yf.ztiles);
+    c->a_yi.ztiles = zt(ny, KC_Y_INV, 1); c->g_yi = grid(nx, c->a_yi.ztiles);
+    c->a_xp.ztiles = zt(nx, KC_XPASS, 1); c->g_xp = grid(ny, c->a_xp.ztiles);
 }
 
 // Fields (jobs) one launch of a strided kernel class processes.

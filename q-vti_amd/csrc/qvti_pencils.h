@@ -64,6 +64,29 @@ __device__ __forceinline__ TileShot tile_shot(int ntiles, int nshots)
     return {t < ntiles ? t : -1, r >> 3};
 }
 
+// (plane, z-tile) of a strided-axis workgroup.  xcd = 0: blockIdx.x =
+// plane * ztiles + tile.  xcd = 1: XCD-aware placement on a 1-D grid of
+// ceil(nplanes/8)*8*ztiles workgroups - the z-tiles of one plane get ids that
+// differ by multiples of 8, so they run on the same XCD (same L2), dispatched
+// close together, and the 128-byte lines that neighbouring tiles share when
+// a tile's rows do not end on a line boundary are fetched from HBM once.
+// plane = -1 for the padding workgroups.
+struct PlaneTile {
+    int plane, zt;
+};
+__device__ __forceinline__ PlaneTile plane_tile(int ztiles, int nplanes, int xcd)
+{
+    const int id = blockIdx.x;
+    if (!xcd) {
+        const int pl = id / ztiles;
+        return {pl, id - pl * ztiles};
+    }
+    const int grp = 8 * ztiles;
+    const int g = id / grp, r = id - g * grp;
+    const int pl = g * 8 + (r & 7);
+    return {pl < nplanes ? pl : -1, r >> 3};
+}
+
 template <class G>
 __device__ __forceinline__ void load_twiddles(float2* tws, const float2* tw)
 {
@@ -90,7 +113,7 @@ struct StridedArgs {
     int nx, ny, nz, pml;
     long long sx, sy;
     int axis;  // 0: x (stride sx), 1: y (stride sy)
-    int ztiles;
+    int ztiles, nplanes, xcd;  // workgroup placement: plane_tile()
     int mode;
     float dt;
     long long shot_stride, slab_stride;
@@ -112,9 +135,11 @@ void k2_deriv_strided(StridedArgs a)
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
-    const int other = blockIdx.x / a.ztiles;
+    const PlaneTile pt = plane_tile(a.ztiles, a.nplanes, a.xcd);
+    if (pt.plane < 0) return;
+    const int other = pt.plane;
     const int p = tid % G::P, jj = tid / G::P;
-    const int zc = (blockIdx.x - other * a.ztiles) * 2 * G::P + 2 * p;
+    const int zc = pt.zt * 2 * G::P + 2 * p;
     const int nz = a.nz;
     const bool zok = zc < nz;
     // columns past nz (last tile of a ragged z) load a valid column and are
@@ -452,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
 struct CStridedArgs {
     int nz;
     long long sx, sy;
-    int axis, ztiles;
+    int axis, ztiles, nplanes, xcd;
     long long shot_stride;
     const float2* tw;
     int nfields;
@@ -465,16 +490,18 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
-    const int other = blockIdx.x / a.ztiles;
+    const PlaneTile pt = plane_tile(a.ztiles, a.nplanes, a.xcd);
+    if (pt.plane < 0) return;
+    const int other = pt.plane;
     const int p = tid % G::P, jj = tid / G::P;
-    const int zc = (blockIdx.x - other * a.ztiles) * G::P + p;
+    const int zc = pt.zt * G::P + p;
     const bool zok = zc < a.nz;
     const bool act1 = jj < G::N2 && zok, act2 = jj < G::N1 && zok;
     const int astride = (int)(a.axis == 0 ? a.sx : a.sy);
     // uniform tile base (shot, plane, first column) + unsigned lane offset
     const long long tbase = (long long)shot * a.shot_stride +
                             (a.axis == 0 ? (long long)other * a.sy : (long long)other * a.sx) +
-                            (long long)(blockIdx.x - other * a.ztiles) * G::P;
+                            (long long)pt.zt * G::P;
     const unsigned voff = (unsigned)(jj * astride + p);
     float2* pen = lds + p * G::PS;
     load_twiddles<G>(tws, a.tw);
@@ -531,7 +558,7 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
 struct XPassArgs {
     int ny, nz;
     long long sx, sy;
-    int ztiles;
+    int ztiles, xcd;  // planes: ny
     long long shot_stride;
     const float2* tw;
     const float *kfx, *kfy, *kfz;
@@ -568,14 +595,16 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
-    const int iy = blockIdx.x / a.ztiles;
+    const PlaneTile pt = plane_tile(a.ztiles, a.ny, a.xcd);
+    if (pt.plane < 0) return;
+    const int iy = pt.plane;
     const int p = tid % G::P, jj = tid / G::P;
-    const int zc = (blockIdx.x - iy * a.ztiles) * G::P + p;
+    const int zc = pt.zt * G::P + p;
     const bool zok = zc < a.nz;
     const bool act1 = jj < G::N2 && zok, act2 = jj < G::N1 && zok;
     // uniform tile base (shot, plane, first column) + unsigned lane offset
     const long long tbase = (long long)shot * a.shot_stride + (long long)iy * a.sy +
-                            (long long)(blockIdx.x - iy * a.ztiles) * G::P;
+                            (long long)pt.zt * G::P;
     const int sxi = (int)a.sx;
     const unsigned voff = (unsigned)(jj * sxi + p);
     float2* pen = lds + p * G::PS;
