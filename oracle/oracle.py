@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle (oracle/qvti_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg as the checker.  The product path never imports
-this module.
+bench.py's cpu_baseline leg, to check the HIP engine's results and to time
+the CPU baseline.  The product path never imports this module.
 """
 import ctypes
 import os

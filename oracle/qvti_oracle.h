@@ -2,8 +2,8 @@
  * qvti_oracle.h -- CPU oracle for the Q-VTI time-stepping loop.
  *
  * TEST INFRASTRUCTURE ONLY.  This is a literal CPU restatement of the
- * reference's per-shot loop (cuviscoelas_3D.cu:1255-1605) used as the parity
- * checker for the HIP engine.  Only tests/, __graft_entry__.smoke() and
+ * reference's per-shot loop (cuviscoelas_3D.cu:1255-1605), against which the
+ * HIP engine's results are checked.  Only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg may load it; the product path never does.
  *
  * It keeps the reference's structure on purpose: nine 3-D C2C forward
