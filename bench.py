@@ -72,7 +72,7 @@ def engine(g, max_shots=1):
                        vti=g["vti"], max_shots=max_shots)
 
 
-def run_plain(name, steps, warmup, kspace_vref=0.0):
+def run_plain(name, steps, warmup, kspace_vref=0.0, tiles=1):
     """Two timed regions of `steps` steps each on one shot, after `warmup`
     steps: (1) the production path (one HIP graph per step, blow-up probe
     every 50 steps) gives the throughput; (2) the same kernels launched
@@ -87,6 +87,8 @@ def run_plain(name, steps, warmup, kspace_vref=0.0):
     e.set_model_dict(m)
     if kspace_vref > 0:
         e.set_kspace(kspace_vref)
+    if tiles != 1:
+        e.set_autotune(tiles)
     setup_s = time.time() - t0
     t0 = time.perf_counter()
     e.begin(nt, rik, [qvti.make_shot(sh["kind"], sh["src"], sh["rx"], sh["ry"], sh["rz"])])
@@ -355,6 +357,8 @@ def main():
     ap.add_argument("--batched", action="store_true", help="32-shot batched leg instead (--workload, default vti160)")
     ap.add_argument("--served", type=int, default=0, metavar="NREQ", help="serving leg with NREQ requests instead")
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--tiles", type=int, default=1, metavar="MODE",
+                    help="tile mode of the strided kernels (qvti_set_autotune; default 1 = autotuned)")
     ap.add_argument("--kspace", type=float, default=0.0, metavar="VREF",
                     help="time the opt-in k-space corrected scheme (reference speed VREF m/s) instead")
     args = ap.parse_args()
@@ -374,7 +378,7 @@ def main():
         print(json.dumps(run_served(args.served)), flush=True)
         return
     name = args.workload
-    g, wall, wall_timed, kt, setup_s, begin_s, tiles, blow_up = run_plain(name, args.steps, args.warmup, args.kspace)
+    g, wall, wall_timed, kt, setup_s, begin_s, tiles, blow_up = run_plain(name, args.steps, args.warmup, args.kspace, args.tiles)
     N = g["nx"] * g["ny"] * g["nz"]
     gcs = N * args.steps / wall / 1e9
     balg = b_alg(g["nx"], g["ny"], g["nz"], g["pml"], g["vti"])
@@ -420,6 +424,8 @@ def main():
     }
     if args.kspace > 0:
         out["config"]["kspace_vref"] = args.kspace
+    if args.tiles != 1:
+        out["config"]["tile_mode"] = args.tiles
     if blow_up is not None:
         out["stability"] = ("the blow-up probe fired by step %d: dt %.0e s is beyond the scheme's stability limit "
                             "for this model (the work per step is unchanged)" % (blow_up, g["dt"]))
