@@ -4,8 +4,9 @@ Default: the plain single-shot path on the 256^3 VTI workload (total grid,
 20-cell C-PML, dx = 12.5 m, dt = 1 ms, seeded smooth random model), W warm-up
 steps then K timed steps.  Prints ONE JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload vti256|vti400|vti128|iso64]
-                  [--batched] [--served] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload vti256|vti400|vti160|vti128|iso64]
+                  [--batched] [--served NREQ] [--tiles MODE] [--kspace VREF] [--no-cpu-baseline]
+                  [--copy-probe]
 
 value = total cells x K / timed wall seconds / 1e9 (Gcell-steps/s) over K
 steps of the production path (HIP graph per step), bracketed by device
