@@ -97,13 +97,14 @@ def test_kspace_tile_widths_and_batching_bitwise(mode):
 
 
 def test_kspace_stabilises_layered_vti_at_1ms():
-    """The layered VTI 128^3 model (vp up to 4500 m/s, eps up to 0.25, Qp down
-    to 30) at dt = 1 ms: c dt / dx reaches 0.55 against the plain scheme's
-    limit 0.37, and the blow-up probe stops the plain run; with vref = 1.25 x
-    the fastest horizontal qP speed (Q dispersion adds ~12% at the grid's
-    highest |k|) the same 300 steps stay finite and bounded (|v| of a unit
-    force source stays far below the probe's limit)."""
-    n, pml, f0, dt, nt = 128, 20, 20.0, 1e-3, 300
+    """The layered VTI 128^3 configuration (vp up to 4500 m/s, eps up to 0.25,
+    Qp down to 30) at its specified dt = 1 ms and full length (2000 steps):
+    c dt / dx reaches 0.55 against the plain scheme's limit 0.37, and the
+    blow-up probe stops the plain run; with vref = 1.25 x the fastest
+    horizontal qP speed (Q dispersion adds ~12% at the grid's highest |k|)
+    the run stays finite and bounded (|v| of a unit force source stays far
+    below the probe's limit)."""
+    n, pml, f0, dt, nt = 128, 20, 20.0, 1e-3, 2000
     m = qvti.make_model(1, 11, (n, n, n), [1800, 4500, 0.25, 30, 200], nlay=6, vti=True)
     vh = float((m["vp"].astype(np.float64) * np.sqrt(1 + 2 * m["eps"].astype(np.float64))).max())
     rik = qvti.ricker(nt, dt, f0)
@@ -130,3 +131,25 @@ def test_set_kspace_rejects_bad_arguments():
             with pytest.raises(qvti.QvtiError) as err:
                 e.set_kspace(*bad)
             assert err.value.code == qvti.E_ARG
+
+
+def test_kspace_full_length_shot_at_256():
+    """The 256^3 smooth VTI configuration as specified: 3000 steps of 1 ms
+    (c dt / dx up to 0.49 before Q dispersion).  With the k-space correction
+    (vref = 1.25 x the fastest horizontal qP speed) the whole shot runs with
+    the blow-up probe armed, and the gathers stay finite and bounded."""
+    import bench
+
+    g, m, sh = bench.make_workload("vti256")
+    nt = 3000
+    vh = float((m["vp"].astype(np.float64) * np.sqrt(1 + 2 * m["eps"].astype(np.float64))).max())
+    rik = qvti.ricker(nt, g["dt"], g["f0"])
+    with bench.engine(g) as e:
+        e.set_model_dict(m)
+        e.set_probe(50, 1e4)
+        e.set_kspace(1.25 * vh)
+        rec = e.forward(nt, rik, qvti.make_shot(sh["kind"], sh["src"], sh["rx"], sh["ry"], sh["rz"]))
+    assert np.isfinite(rec).all()
+    peak = np.abs(rec).max()
+    print("256^3, 3000 steps, k-space mode: peak |v| at the receivers %.3e" % peak)
+    assert 0 < peak < 1e2
