@@ -117,10 +117,14 @@ int qvti_forward_acoustic_3D(qvti_ctx *c, int myid, int is, int nt, int ntx, int
                              int rny_max, int dr, float *record, float *record2, float *record3);
 
 /* Snapshot hook: every `every` steps (it % every == 0, after the step's
- * source injection) copy the interior window [pml, n-pml)^3 of `field`,
- * ix slowest, into host_buf (capacity in floats).  every = 0 disables.
+ * source injection) copy the snapshot window (default: the interior
+ * [pml, n-pml)^3) of `field`, ix slowest, into host_buf (capacity in floats).  every = 0 disables.
  * The reference's disabled dump (cuviscoelas_3D.cu:1615-1627) used 200. */
 int qvti_set_snapshot(qvti_ctx *c, int field, int every, float *host_buf, long long capacity);
+/* Snapshot window: the box [lo, hi) of total-grid indices (x, y, z) that the
+ * snapshots copy, ix slowest.  Default: the interior [pml, n - pml) on every
+ * axis, as the reference's dump (cuviscoelas_3D.cu:1615-1627). */
+int qvti_set_snapshot_window(qvti_ctx *c, const int *lo, const int *hi);
 
 /* Release everything.  Replaces cuda_Device_free (Myfunctions.h:98-102). */
 int qvti_destroy(qvti_ctx *c);

@@ -34,21 +34,31 @@ __global__ __launch_bounds__(256) void k_record(RecArgs a)
 
 __global__ void k_tick(int* it) { *it = *it + 1; }
 
-// Interior-window snapshot of one field when it % every == 0.
-__global__ __launch_bounds__(256) void k_snapshot(const float* f, int nx, int ny, int nz, int pml, int every,
-                                                       const int* it, float* out, long long cap)
+// Snapshot of one field over the box [lo, hi) (ix slowest) when
+// it % every == 0; snapshot k lands at out[k * W], W the box's cell count.
+struct SnapArgs {
+    const float* f;
+    int ny, nz;
+    int lo[3], hi[3];
+    int every;
+    const int* it;
+    float* out;
+    long long cap;
+};
+
+__global__ __launch_bounds__(256) void k_snapshot(SnapArgs a)
 {
-    const int t = *it;
-    if (every <= 0 || t % every != 0) return;
-    const long long k = t / every;
-    const int wx = nx - 2 * pml, wy = ny - 2 * pml, wz = nz - 2 * pml;
+    const int t = *a.it;
+    if (a.every <= 0 || t % a.every != 0) return;
+    const long long k = t / a.every;
+    const int wx = a.hi[0] - a.lo[0], wy = a.hi[1] - a.lo[1], wz = a.hi[2] - a.lo[2];
     const long long W = (long long)wx * wy * wz;
-    if ((k + 1) * W > cap) return;
+    if ((k + 1) * W > a.cap) return;
     for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < W; q += (long long)gridDim.x * 256) {
         const int ix = (int)(q / ((long long)wy * wz));
         const long long rem = q - (long long)ix * wy * wz;
         const int iy = (int)(rem / wz), iz = (int)(rem - (long long)iy * wz);
-        out[k * W + q] = f[((long long)(ix + pml) * ny + iy + pml) * nz + iz + pml];
+        a.out[k * W + q] = a.f[((long long)(ix + a.lo[0]) * a.ny + iy + a.lo[1]) * a.nz + iz + a.lo[2]];
     }
 }
 

@@ -230,8 +230,9 @@ struct qvti_ctx {
     long long rec_cap = 0;
     int probe_every = 50;
     float probe_limit = 1e30f;
-    // snapshot
+    // snapshot of the box [snap_lo, snap_hi) (default: the interior window)
     int snap_field = 0, snap_every = 0;
+    int snap_lo[3] = {0, 0, 0}, snap_hi[3] = {0, 0, 0};
     float* snap_host = nullptr;
     long long snap_cap = 0;
     float* d_snap = nullptr;
@@ -361,6 +362,10 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
     c->g = *g;
     c->S = max_shots;
     c->N = (long long)g->nx * g->ny * g->nz;
+    for (int a = 0; a < 3; ++a) {
+        c->snap_lo[a] = g->pml;
+        c->snap_hi[a] = dims[a] - g->pml;
+    }
     c->nrows = (long long)g->nx * g->ny;
     int rc = QVTI_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -943,6 +948,13 @@ static int autotune_tiles(qvti_ctx* c)
     return rc;
 }
 
+// Cells of the snapshot box.
+static long long snap_cells(const qvti_ctx* c)
+{
+    return (long long)(c->snap_hi[0] - c->snap_lo[0]) * (c->snap_hi[1] - c->snap_lo[1]) *
+           (c->snap_hi[2] - c->snap_lo[2]);
+}
+
 static const float* snap_source(qvti_ctx* c)
 {
     const int f = c->snap_field;
@@ -956,8 +968,15 @@ static void enqueue_step_with_snap(qvti_ctx* c)
     enqueue_step(c);
     if (c->snap_every > 0 && c->d_snap) {
         const int e0 = c->timing ? record_event(c, KC_SNAP) : -1;
-        hipLaunchKernelGGL(k_snapshot, dim3(256), dim3(kThreads), 0, c->stream, snap_source(c), c->g.nx, c->g.ny,
-                           c->g.nz, c->g.pml, c->snap_every, (const int*)c->d_it, c->d_snap, c->d_snap_cap);
+        SnapArgs sa;
+        sa.f = snap_source(c);
+        sa.ny = c->g.ny; sa.nz = c->g.nz;
+        for (int a = 0; a < 3; ++a) { sa.lo[a] = c->snap_lo[a]; sa.hi[a] = c->snap_hi[a]; }
+        sa.every = c->snap_every;
+        sa.it = c->d_it;
+        sa.out = c->d_snap;
+        sa.cap = c->d_snap_cap;
+        hipLaunchKernelGGL(k_snapshot, dim3(256), dim3(kThreads), 0, c->stream, sa);
         if (e0 >= 0) record_event(c, -1 - KC_SNAP);
     }
 }
@@ -1057,7 +1076,7 @@ extern "C" int qvti_begin(qvti_ctx* c, int nt, const float* rik, const qvti_shot
     HIPCHK(hipMemcpyAsync(c->d_probe, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
     // snapshots
     if (c->snap_every > 0) {
-        const long long W = (long long)(g.nx - 2 * g.pml) * (g.ny - 2 * g.pml) * (g.nz - 2 * g.pml);
+        const long long W = snap_cells(c);
         const long long nsnap = (nt + c->snap_every - 1) / c->snap_every;
         const long long need = std::min(W * nsnap, c->snap_cap);
         if (need > c->d_snap_cap) {
@@ -1168,8 +1187,7 @@ extern "C" int qvti_finish(qvti_ctx* c, float* rec_vx, float* rec_vy, float* rec
         if (outs[k] && n > 0)
             HIPCHK(hipMemcpy(outs[k], c->d_rec + k * n, sizeof(float) * n, hipMemcpyDeviceToHost));
     if (c->snap_every > 0 && c->d_snap && c->snap_host) {
-        const qvti_grid& g = c->g;
-        const long long W = (long long)(g.nx - 2 * g.pml) * (g.ny - 2 * g.pml) * (g.nz - 2 * g.pml);
+        const long long W = snap_cells(c);
         const long long nsnap = (c->steps_done - 1) / c->snap_every + 1;
         const long long cnt = std::min(W * nsnap, c->d_snap_cap);
         HIPCHK(hipMemcpy(c->snap_host, c->d_snap, sizeof(float) * cnt, hipMemcpyDeviceToHost));
@@ -1186,6 +1204,7 @@ namespace {
 struct CkptHeader {
     char magic[8];
     int version, nx, ny, nz, pml, vti, nshots, nt, steps_done, total_rec, snap_every, snap_field, probe;
+    int snap_lo[3], snap_hi[3];
     long long nfloats;
 };
 const char kCkptMagic[8] = {'Q', 'V', 'T', 'I', 'C', 'K', 'P', '1'};
@@ -1204,8 +1223,7 @@ std::vector<std::pair<float*, long long>> ckpt_regions(qvti_ctx* c)
     }
     if (c->total_rec > 0) r.push_back({c->d_rec, 3LL * c->total_rec * c->nt});
     if (c->snap_every > 0 && c->d_snap && c->steps_done > 0) {
-        const qvti_grid& g = c->g;
-        const long long W = (long long)(g.nx - 2 * g.pml) * (g.ny - 2 * g.pml) * (g.nz - 2 * g.pml);
+        const long long W = snap_cells(c);
         const long long taken = (c->steps_done - 1) / c->snap_every + 1;
         r.push_back({c->d_snap, std::min(W * taken, c->d_snap_cap)});
     }
@@ -1221,6 +1239,7 @@ CkptHeader ckpt_header(qvti_ctx* c, int probe)
     h.nx = c->g.nx; h.ny = c->g.ny; h.nz = c->g.nz; h.pml = c->g.pml; h.vti = c->g.vti;
     h.nshots = c->nshots; h.nt = c->nt; h.steps_done = c->steps_done; h.total_rec = c->total_rec;
     h.snap_every = c->snap_every; h.snap_field = c->snap_field; h.probe = probe;
+    for (int a = 0; a < 3; ++a) { h.snap_lo[a] = c->snap_lo[a]; h.snap_hi[a] = c->snap_hi[a]; }
     for (auto& p : ckpt_regions(c)) h.nfloats += p.second;
     return h;
 }
@@ -1264,7 +1283,8 @@ extern "C" int qvti_restore(qvti_ctx* c, const void* buf, long long size)
     if (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1) return qvti_fail(QVTI_E_ARG, "not a checkpoint image");
     if (h.nx != c->g.nx || h.ny != c->g.ny || h.nz != c->g.nz || h.pml != c->g.pml || h.vti != c->g.vti ||
         h.nshots != c->nshots || h.nt != c->nt || h.total_rec != c->total_rec || h.snap_every != c->snap_every ||
-        h.snap_field != c->snap_field)
+        h.snap_field != c->snap_field || memcmp(h.snap_lo, c->snap_lo, sizeof h.snap_lo) != 0 ||
+        memcmp(h.snap_hi, c->snap_hi, sizeof h.snap_hi) != 0)
         return qvti_fail(QVTI_E_ARG, "checkpoint of a different run (grid, shots, nt, receivers or snapshots differ)");
     if (h.steps_done < 0 || h.steps_done > h.nt) return qvti_fail(QVTI_E_ARG, "checkpoint step %d out of range", h.steps_done);
     HIPCHK(hipSetDevice(c->device));
@@ -1365,6 +1385,21 @@ extern "C" int qvti_set_snapshot(qvti_ctx* c, int field, int every, float* host_
     c->snap_host = host_buf;
     c->snap_cap = capacity;
     c->begun = 0;  // a new qvti_begin is needed (the step graph changes)
+    return QVTI_OK;
+}
+
+extern "C" int qvti_set_snapshot_window(qvti_ctx* c, const int* lo, const int* hi)
+{
+    if (!c || !lo || !hi) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    const int n[3] = {c->g.nx, c->g.ny, c->g.nz};
+    for (int a = 0; a < 3; ++a)
+        if (lo[a] < 0 || lo[a] >= hi[a] || hi[a] > n[a])
+            return qvti_fail(QVTI_E_ARG, "snapshot window axis %d: [%d, %d) not inside [0, %d)", a, lo[a], hi[a], n[a]);
+    for (int a = 0; a < 3; ++a) {
+        c->snap_lo[a] = lo[a];
+        c->snap_hi[a] = hi[a];
+    }
+    c->begun = 0;
     return QVTI_OK;
 }
 

@@ -28,7 +28,7 @@ EXPORTS = [
     "qvti_server_create", "qvti_server_submit", "qvti_server_poll", "qvti_server_wait",
     "qvti_server_cancel", "qvti_server_clock", "qvti_server_destroy", "qvti_set_autotune", "qvti_tile_choice",
     "qvti_pad_model", "qvti_load_model_raw", "qvti_write_raw", "qvti_set_kspace",
-    "qvti_checkpoint_bytes", "qvti_checkpoint", "qvti_restore",
+    "qvti_checkpoint_bytes", "qvti_checkpoint", "qvti_restore", "qvti_set_snapshot_window",
 ]
 
 
@@ -102,6 +102,7 @@ def lib():
             "qvti_write_raw": [ctypes.c_char_p, vp, ll],
             "qvti_set_kspace": [vp, f, f, f, f],
             "qvti_checkpoint_bytes": [vp], "qvti_checkpoint": [vp, vp, ll], "qvti_restore": [vp, vp, ll],
+            "qvti_set_snapshot_window": [vp, vp, vp],
         }
         for name, args in sig.items():
             if not hasattr(L, name):
@@ -321,11 +322,19 @@ class Engine:
     def set_probe(self, every, limit=1e30):
         check(lib().qvti_set_probe(self._c, every, limit))
 
-    def set_snapshot(self, field, every, nt):
+    def set_snapshot(self, field, every, nt, window=None):
+        """Snapshots of `field` every `every` steps over window = ((x0, y0, z0),
+        (x1, y1, z1)) (default: the interior [pml, n - pml)^3); returns the
+        [nsnap][wx][wy][wz] array they are written to by finish()."""
         g = self.grid
-        W = (g.nx - 2 * g.pml) * (g.ny - 2 * g.pml) * (g.nz - 2 * g.pml)
+        if window is None:
+            lo, hi = [g.pml] * 3, [g.nx - g.pml, g.ny - g.pml, g.nz - g.pml]
+        else:
+            lo, hi = list(window[0]), list(window[1])
+        lo_a, hi_a = np.array(lo, np.int32), np.array(hi, np.int32)
+        check(lib().qvti_set_snapshot_window(self._c, _p(lo_a), _p(hi_a)))
         ns = (nt + every - 1) // every if every else 0
-        self._snap = np.zeros((max(ns, 1), g.nx - 2 * g.pml, g.ny - 2 * g.pml, g.nz - 2 * g.pml), np.float32)
+        self._snap = np.zeros((max(ns, 1),) + tuple(int(h - l) for l, h in zip(lo, hi)), np.float32)
         check(lib().qvti_set_snapshot(self._c, field, every, _p(self._snap) if every else None,
                                       self._snap.size if every else 0))
         return self._snap

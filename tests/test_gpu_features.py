@@ -159,3 +159,33 @@ def test_checkpoint_restore_continues_bit_identically():
         with pytest.raises(qvti.QvtiError) as err:
             e.restore(img)
         assert err.value.code == qvti.E_ARG
+
+
+def test_snapshot_window_reaches_into_the_cpml():
+    """qvti_set_snapshot_window: a box that includes C-PML cells; its last
+    snapshot (taken after the final step) matches the oracle's final field,
+    and its interior part equals the default interior-window snapshots."""
+    shot, model = small_case(24, 20, 32, 4, 31, O.SRC_EXPLOSIVE, True)
+    nt, every, field = shot["nt"], 5, 3
+    lo, hi = (0, 2, 1), (24, 17, 32)
+    ref = O.run(shot, model, "f64", state=True)["state"][field]
+    with engine_for(shot, model) as e:
+        box = e.set_snapshot(field, every, nt, window=(lo, hi))
+        e.begin(nt, shot["rik"], [shot_of(shot)])
+        e.step(nt)
+        e.finish()
+        box = box.copy()
+        inner = e.set_snapshot(field, every, nt)
+        e.begin(nt, shot["rik"], [shot_of(shot)])
+        e.step(nt)
+        e.finish()
+    want = ref[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+    assert (nt - 1) % every == 0 and box.shape == (7,) + want.shape
+    assert rel_l2(box[-1], want) < TOL
+    p, n = shot["pml"], (shot["nx"], shot["ny"], shot["nz"])
+    part = box[:, p - lo[0]:n[0] - p - lo[0], p - lo[1]:n[1] - p - lo[1], p - lo[2]:n[2] - p - lo[2]]
+    assert np.array_equal(part, inner)
+    with engine_for(shot, model) as e2:
+        with pytest.raises(qvti.QvtiError) as err:
+            e2.set_snapshot(field, every, nt, window=((0, 0, 0), (25, 20, 32)))
+        assert err.value.code == qvti.E_ARG
