@@ -45,7 +45,22 @@ def build(verbose=False, out=None, defines=()):
     tmp = lib + ".tmp"
     _run([hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, lib)
+    build_host(lib, verbose)
     return lib
+
+
+HOST = os.path.join(HERE, "qvti_shots")
+
+
+def build_host(lib=LIB, verbose=False):
+    """The C++ host driver (host/qvti_shots.cpp) linked against the library;
+    it finds libqvti.so next to itself at run time."""
+    tmp = HOST + ".tmp"
+    _run([hipcc(), "-x", "c++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(HERE, "host", "qvti_shots.cpp"), "-o", tmp, "-L", os.path.dirname(lib), "-lqvti",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, HOST)
+    return HOST
 
 
 def _run(cmd, verbose):

@@ -1,0 +1,56 @@
+"""The C++ host driver (q-vti_amd/qvti_shots, host/qvti_shots.cpp): the
+reference host's forward-modelling main on one GPU.  A 2-shot run over a raw
+interior vp file is checked against the FP32 / FP64 oracle on the model and
+geometry the reference host builds (edge-replicated vp, vs = vp/1.5, Qp 200,
+Qs 150, rho 1800; x-force at s_iz + 10; receivers every dr-th interior cell)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import qvti
+from cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXE = os.path.join(os.path.dirname(HERE), "q-vti_amd", "qvti_shots")
+
+
+def test_host_driver_matches_the_oracle(tmp_path):
+    nx, ny, nz, pml, nt, dr, dt, f0, d = 24, 12, 16, 4, 40, 2, 5e-4, 30.0, 10.0
+    rng = np.random.default_rng(5)
+    vp_int = (2600 + 600 * rng.random((nx, ny, nz))).astype(np.float32)
+    vp_file = str(tmp_path / "vp.bin")
+    qvti.write_raw(vp_file, vp_int)
+    args = [EXE, "--nx", nx, "--ny", ny, "--nz", nz, "--pml", pml, "--nt", nt, "--dx", d, "--dy", d, "--dz", d,
+            "--dt", dt, "--f0", f0, "--nsx", 2, "--dsx", 5, "--nsx0", 8, "--nsy0", 6, "--depth", 2, "--dr", dr,
+            "--vp", vp_file, "--out", str(tmp_path)]
+    r = subprocess.run([str(a) for a in args], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    R = (nx // dr) * (ny // dr)
+    recs = [[np.fromfile(str(tmp_path / ("%s_shot%d.bin" % (c, s))), np.float32).reshape(nt, R)
+             for c in ("vx", "vy", "vz")] for s in range(2)]
+    assert open(str(tmp_path / "source_xy.txt")).read().split() == ["8", "6", "13", "6"]
+
+    ntx, nty, ntz = nx + 2 * pml, ny + 2 * pml, nz + 2 * pml
+    vp = qvti.pad_model(vp_int, pml)
+    model = dict(vp=vp, vs=(vp / np.float32(1.5)).astype(np.float32), rho=np.full(vp.shape, 1800, np.float32),
+                 gp=qvti.gamma_from_q(np.full(vp.shape, 200, np.float32)),
+                 gs=qvti.gamma_from_q(np.full(vp.shape, 150, np.float32)))
+    rx, ry = qvti.reference_receivers(pml, nx // dr, ny // dr, dr)
+    w0 = float(np.float32(2 * np.float32(3.1415926) * np.float32(f0)))
+    for s, sx in enumerate((pml + 8, pml + 13)):
+        shot = dict(nx=ntx, ny=nty, nz=ntz, pml=pml, nt=nt, dx=d, dy=d, dz=d, dt=dt, f0=f0, w0=w0,
+                    src_kind=O.SRC_FX, sx=sx, sy=pml + 6, sz=pml + 2 + 10, rx=rx, ry=ry,
+                    rz=np.full(R, pml + 2, np.int32), rik=qvti.ricker(nt, dt, f0))
+        r32 = O.run(shot, model, "f32")["records"]
+        r64 = O.run(shot, model, "f64")["records"]
+        for c in range(3):
+            floor = rel_l2(r32[c], r64[c])
+            e32, e64 = rel_l2(recs[s][c], r32[c]), rel_l2(recs[s][c], r64[c])
+            print("shot %d component %d: vs FP32 %.2e, vs FP64 %.2e (floor %.2e)" % (s, c, e32, e64, floor))
+            assert e64 < max(1e-5, 2 * floor) and e32 < max(1e-5, 3 * floor), (s, c)

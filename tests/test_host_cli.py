@@ -1,0 +1,21 @@
+"""CPU checks of the C++ host driver's command line (no device work)."""
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EXE = os.path.join(os.path.dirname(HERE), "q-vti_amd", "qvti_shots")
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="host driver not built")
+def test_help_and_argument_errors():
+    r = subprocess.run([EXE, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "usage:" in r.stdout
+    r = subprocess.run([EXE, "--nx", "8", "--bogus", "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "unknown option --bogus" in r.stderr
+    r = subprocess.run([EXE, "--ny", "8"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "must be positive" in r.stderr
+    r = subprocess.run([EXE, "--nx", "8", "--ny", "8", "--nz", "8", "--eps", "e.bin"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and "go together" in r.stderr
