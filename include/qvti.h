@@ -118,7 +118,8 @@ int qvti_forward_acoustic_3D(qvti_ctx *c, int myid, int is, int nt, int ntx, int
 
 /* Snapshot hook: every `every` steps (it % every == 0, after the step's
  * source injection) copy the snapshot window (default: the interior
- * [pml, n-pml)^3) of `field`, ix slowest, into host_buf (capacity in floats).  every = 0 disables.
+ * [pml, n-pml)^3) of `field`, ix slowest, into host_buf (capacity in
+ * floats).  every = 0 disables.
  * The reference's disabled dump (cuviscoelas_3D.cu:1615-1627) used 200. */
 int qvti_set_snapshot(qvti_ctx *c, int field, int every, float *host_buf, long long capacity);
 /* Snapshot window: the box [lo, hi) of total-grid indices (x, y, z) that the
