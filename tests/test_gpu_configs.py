@@ -85,12 +85,13 @@ def test_config2_vti128_layered_vs_golden():
     assert e64 < max(1e-5, 2 * floor) and e32 < max(1e-5, 3 * floor)
 
 
-@pytest.mark.parametrize("workload,nt", [("vti256", 3), ("vti400", 2)])
+@pytest.mark.parametrize("workload,nt", [("vti256", 3), ("vti400", 2), ("vti160", 4)])
 def test_config3_full_size_window(workload, nt):
-    """The 256^3 and 400x400x200 single-shot configurations (seeded smooth
-    random VTI model, Q in [20,150], eps/delta co-varying with vp, 20-cell
-    C-PML, dx 12.5 m, dt 1 ms): the first steps at full size against the
-    oracle, on a receiver line through the source."""
+    """The 256^3 and 400x400x200 single-shot configurations and the batched
+    configuration's 160x160x128 grid (seeded smooth random VTI model, Q in
+    [20,150], eps/delta co-varying with vp, 20-cell C-PML, dx 12.5 m, dt 1 ms):
+    the first steps at full size against the oracle, on a receiver line
+    through the source."""
     import bench
 
     g, m, _ = bench.make_workload(workload)

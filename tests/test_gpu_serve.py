@@ -85,6 +85,35 @@ def test_served_records_match_the_oracle():
     assert rel_l2(rec[1][:, :, la:], ref["records"][:, :, la:]) < 1e-5
 
 
+def test_served_request_on_a_serving_grid_matches_the_oracle():
+    """A request on the serving stream's smallest grid (96^3, isotropic, Q 20
+    to 200 drawn as in the stream, dx 12.5 m, dt 0.8 ms), served next to a
+    VTI request: its first 12 steps against the FP32 / FP64 oracle."""
+    n, pml, nt = 96, 20, 12
+    m = qvti.make_model(2, 21, (n, n, n), [2000, 4000, 0.0, 20, 200, 15, 0.0])
+    model = {k: m[k] for k in ("vp", "gp", "vs", "gs", "rho")}
+    f0, dt, d = 20.0, 8e-4, 12.5
+    w0 = float(np.float32(2 * 3.1415926 * f0))
+    rx = np.arange(30, 66, 3)
+    shot = dict(nx=n, ny=n, nz=n, pml=pml, nt=nt, dx=d, dy=d, dz=d, dt=dt, f0=f0, w0=w0, src_kind=O.SRC_FZ,
+                sx=48, sy=48, sz=pml + 1, rx=rx, ry=np.full(len(rx), 48), rz=np.full(len(rx), pml + 1),
+                rik=qvti.ricker(nt, dt, f0))
+    grid = qvti.Grid(n, n, n, pml, d, d, d, dt, f0, w0, 0)
+    sh = qvti.make_shot(O.SRC_FZ, (48, 48, pml + 1), shot["rx"], shot["ry"], shot["rz"])
+    with qvti.Server(1 << 40, max_active=2, steps_per_slice=5) as s:
+        g2, shot2, model2, sh2 = build(REQS[1])
+        other = s.submit(g2, shot2["nt"], shot2["rik"], model2, sh2)
+        i = s.submit(grid, nt, shot["rik"], model, sh)
+        assert s.wait(i, 300.0) == 0 and s.wait(other, 300.0) == 0
+        rc, _, rec = s.result(i)
+        assert rc == 0
+    r32 = O.run(shot, model, "f32")["records"]
+    r64 = O.run(shot, model, "f64")["records"]
+    e32, e64 = rel_l2(rec, r32), rel_l2(rec, r64)
+    print("served 96^3 request, 12 steps: vs FP32 oracle %.2e, vs FP64 oracle %.2e" % (e32, e64))
+    assert e32 < 1e-5 and e64 < 1e-5
+
+
 def test_budget_queues_and_rejects():
     # room for exactly one context at a time: requests of different grids
     # are admitted one after another, evicting the pooled context
