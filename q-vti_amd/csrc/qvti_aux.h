@@ -6,8 +6,8 @@
 namespace qvti {
 
 // Receivers: rec[it][r] = v[cell_r] for the three components (after the
-// source injection), cuviscoelas_3D.cu:782-792.  Also advances nothing: the
-// step counter is advanced by k_tick at the start of the next step.
+// source injection), cuviscoelas_3D.cu:782-792.  The step counter `it`
+// advances in the first kernel of the next step (k2_deriv_strided's tick).
 struct RecArgs {
     const float *vx, *vy, *vz;
     long long shot_stride;
@@ -32,7 +32,6 @@ __global__ __launch_bounds__(256) void k_record(RecArgs a)
     a.rz[o] = a.vz[ip];
 }
 
-__global__ void k_tick(int* it) { *it = *it + 1; }
 
 // Snapshot of one field over the box [lo, hi) (ix slowest) when
 // it % every == 0; snapshot k lands at out[k * W], W the box's cell count.

@@ -125,6 +125,8 @@ struct StridedArgs {
     // blocked group of nf fields per shot ([shot][N/64][nf][64], see
     // blk_index), each job's out pointing at its field (group base + f*64)
     int bnf;
+    // step counter advanced by this launch (the first kernel of a step), or null
+    int* tick;
     DJob job[15];
 };
 
@@ -135,6 +137,9 @@ void k2_deriv_strided(StridedArgs a)
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
+    // one thread of the grid advances the step counter; the kernels of the
+    // step that read it run after this one
+    if (a.tick != nullptr && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid == 0) *a.tick = *a.tick + 1;
     const PlaneTile pt = plane_tile(a.ztiles, a.nplanes, a.xcd);
     if (pt.plane < 0) return;
     const int other = pt.plane;
