@@ -1,12 +1,14 @@
 // qvti_pencils.h -- pencil kernels of the time step on the two-level register
-// FFT (qvti_fft2l.h).  Every kernel is a template over the split G = L2<N1,N2>
-// of its axis length and runs 256 threads.
+// FFT (qvti_fft2l.h).  Every kernel is a template over the split G = L2<N1,N2,PW>
+// of its axis length; the row (z) kernels run 256 threads, the strided-axis
+// (x, y) kernels 256*PW (the tile width the autotuner picks).
 //
 // Per step (reference loop: cuviscoelas_3D.cu:1255-1605):
-//   velocity half-step   k2_deriv_strided(x, VEL_FIRST) -> k2_deriv_strided(y, VEL_ACC)
+//   velocity half-step   k2_deriv_strided(x, VEL_FIRST; advances the step counter)
+//                        -> k2_deriv_strided(y, VEL_ACC)
 //                        -> k2_vel_z (z-derivatives, velocity update, forward z-FFT of v + i dv)
 //   stress half-step     k2_cfft_strided(y, fwd) -> k2_xpass -> k2_cfft_strided(y, inv)
-//                        -> k2_zrows_inv -> k2_deriv_strided(x, STORE) -> k2_deriv_strided(y, STORE)
+//                        -> k2_zrows (inverse z) -> k2_deriv_strided(x, STORE) -> k2_deriv_strided(y, STORE)
 //                        -> k2_stress_z (z-derivatives, stress update, source injection)
 //   receivers            k_record
 //
