@@ -1300,13 +1300,20 @@ extern "C" int qvti_restore(qvti_ctx* c, const void* buf, long long size)
         return qvti_fail(QVTI_E_ARG, "checkpoint image size does not match the run");
     }
     const char* p = static_cast<const char*>(buf) + sizeof h;
+    bool ok = true;
     for (auto& r : regions) {
-        HIPCHK(hipMemcpy(r.first, p, 4 * r.second, hipMemcpyHostToDevice));
+        ok = ok && hipMemcpy(r.first, p, 4 * r.second, hipMemcpyHostToDevice) == hipSuccess;
         p += 4 * r.second;
     }
     const int it = h.steps_done - 1;  // vel_x advances it at the start of each step
-    HIPCHK(hipMemcpy(c->d_it, &it, sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_probe, &h.probe, sizeof(int), hipMemcpyHostToDevice));
+    ok = ok && hipMemcpy(c->d_it, &it, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->d_probe, &h.probe, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        // a partly loaded state must not be stepped: the run has to be begun again
+        c->steps_done = 0;
+        c->begun = 0;
+        return qvti_fail(QVTI_E_DEVICE, "checkpoint upload failed");
+    }
     return QVTI_OK;
 }
 
