@@ -69,6 +69,9 @@ CASES = [
     (24, 20, 16, 4, 40, O.SRC_FZ, True),
     (32, 24, 20, 5, 30, O.SRC_EXPLOSIVE, True),
     (40, 16, 48, 4, 20, O.SRC_FY, True),
+    # the longest supported axis (400 = 20 x 20) on the row axis and on a strided axis
+    (16, 20, 400, 4, 12, O.SRC_EXPLOSIVE, True),
+    (400, 16, 20, 4, 12, O.SRC_FZ, False),
 ]
 
 
