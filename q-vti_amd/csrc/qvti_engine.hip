@@ -635,12 +635,17 @@ static int build_launch_config(qvti_ctx* c)
     // shot in bufA (x) and bufB (y); job k writes field k of the group
     float* DX[15];
     float* DY[15];
-    for (int k = 0; k < 15; ++k) { DX[k] = c->bufA + (long long)k * kBlk; DY[k] = c->bufB + (long long)k * kBlk; }
+    // x-derivatives: three blocked groups of 5 (ex, gx, hx), each [shot][N/64][5][64];
+    // y-derivatives: one blocked group of 15
+    for (int k = 0; k < 15; ++k) {
+        DX[k] = c->bufA + (long long)(k / 5) * 5 * c->S * N + (long long)(k % 5) * kBlk;
+        DY[k] = c->bufB + (long long)k * kBlk;
+    }
     StridedArgs& sxa = c->a_sx;
     strided_common(sxa, 0, Px);
     sxa.mode = MODE_STORE;
     sxa.njobs = 15;
-    sxa.bnf = kG1;
+    sxa.bnf = 5;
     for (int j = 0; j < 5; ++j) {
         sxa.job[j] = job(fam[0][j], DX[j], nullptr, nullptr, 0, 0);          // ex: dx vx-family, D+
         sxa.job[5 + j] = job(fam[2][j], DX[5 + j], nullptr, nullptr, 1, 0);   // gx: dx vz-family, D-
@@ -671,6 +676,7 @@ static int build_launch_config(qvti_ctx* c)
     for (int k = 0; k < 3; ++k)
         for (int j = 0; j < 5; ++j) sz.fam[k][j] = fam[k][j];
     sz.dxb = c->bufA;
+    sz.dx_group = 5LL * c->S * N;
     sz.dyb = c->bufB;
     sz.g1 = c->coef;
     sz.g2 = g.vti ? c->coef + (long long)kG1 * N : nullptr;

@@ -843,9 +843,11 @@ struct StressZArgs {
     const float* fam[3][5];
     float* vel[3];  // vx vy vz: force sources are added here
     // x- and y-derivatives, blocked groups of 15 per shot:
-    //   dxb: ex 0-4 (dx vx-family), gx 5-9 (dx vz-family), hx 10-14 (dx vy-family)
+    //   dxb: three groups of 5, dx_group floats apart: ex (dx vx-family),
+    //        gx (dx vz-family), hx (dx vy-family), each [shot][N/64][5][64]
     //   dyb: ey 0-4 (dy vy-family), gy 5-9 (dy vz-family), hy 10-14 (dy vx-family)
     const float *dxb, *dyb;
+    long long dx_group;
     // coefficient groups G1 {P, S, S-bar} and G2 {H, D} (VTI; else null)
     const float *g1, *g2;
     float* sig[6];  // pxx pyy pzz pxy pxz pyz
@@ -947,7 +949,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
     // uniform block bases; per-cell offsets below are unsigned 32-bit
     const long long cbase = r0 * n;          // flat index of the tile's first cell
     const long long fbase = soff + cbase;    // per-shot natural fields
-    const float* __restrict__ dxs = a.dxb + (long long)shot * kG1 * a.shot_stride;
+    const float* __restrict__ dxs = a.dxb + (long long)shot * 5 * a.shot_stride;
     const float* __restrict__ dys = a.dyb + (long long)shot * kG1 * a.shot_stride;
     constexpr bool vti = VTI;  // G2 (H, D sets) present: a compile-time choice keeps the loads branch-free
     const long long sxb = (long long)shot * a.slabx_stride, syb = (long long)shot * a.slaby_stride,
@@ -969,7 +971,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
             // fields at immediate offsets f*64 floats
             const unsigned ip = (unsigned)cbase + o;
             const unsigned q15 = (ip >> 6) * (kBlk * kG1) + (ip & 63u);
-            const float* __restrict__ dxp = dxs + q15;
+            const float* __restrict__ dxp = dxs + ((ip >> 6) * (kBlk * 5) + (ip & 63u));
             const float* __restrict__ dyp = dys + q15;
             const float* __restrict__ g1p = a.g1 + q15;
             const float* __restrict__ g2p = vti ? a.g2 + ((ip >> 6) * (kBlk * kG2) + (ip & 63u)) : nullptr;
@@ -979,7 +981,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
 #pragma unroll
             for (int jf = 0; jf < 5; ++jf) {
                 const float ez = S(fz[jf]), gz = S(fgx[jf]), hz = S(fhy[jf]);
-                const float ex = dxp[jf * kBlk], gx = dxp[(5 + jf) * kBlk], hx = dxp[(10 + jf) * kBlk];
+                const float ex = dxp[jf * kBlk], gx = dxp[a.dx_group + jf * kBlk], hx = dxp[2 * a.dx_group + jf * kBlk];
                 const float ey = dyp[jf * kBlk], gy = dyp[(5 + jf) * kBlk], hy = dyp[(10 + jf) * kBlk];
                 const float c33 = g1p[jf * kBlk], cs = g1p[(5 + jf) * kBlk], csb = g1p[(10 + jf) * kBlk];
                 float c11 = c33, c13 = c33;
