@@ -2,11 +2,13 @@
 
 Default: the plain single-shot path on the 256^3 VTI workload (total grid,
 20-cell C-PML, dx = 12.5 m, dt = 1 ms, seeded smooth random model), W warm-up
-steps then K timed steps.  Prints ONE JSON line.
+steps then K timed steps, followed by the other legs of the metric - the
+32-shot batched configuration, a seeded Poisson serving stream, a stream-copy
+bandwidth probe - and the CPU baseline.  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload vti256|vti400|vti160|vti128|iso64]
                   [--batched] [--served NREQ] [--tiles MODE] [--kspace VREF] [--no-cpu-baseline]
-                  [--copy-probe]
+                  [--headline-only] [--copy-probe]
 
 value = total cells x K / timed wall seconds / 1e9 (Gcell-steps/s) over K
 steps of the production path (HIP graph per step), bracketed by device
@@ -14,7 +16,8 @@ synchronisations.  A second region of K steps launches the same kernels
 directly with every launch bracketed by HIP events on the engine's stream;
 the dominant kernel's average duration there gives the achieved algorithmic
 GB/s of the roofline object.  roofline.traffic is the measured HBM bytes per
-launch of that kernel from the committed rocprofv3 counter summary.
+launch of that kernel from the committed rocprofv3 counter summary, used only
+while the kernel sources still hash to the ones that summary was measured on.
 """
 import argparse
 import json
@@ -126,17 +129,44 @@ def run_plain(name, steps, warmup, kspace_vref=0.0, tiles=1):
     return g, wall, wall_timed, kt, setup_s, begin_s, tiles, (blow_up[0] if blow_up else None)
 
 
+KERNEL_SOURCES = ("qvti_engine.hip", "qvti_pencils.h", "qvti_fft2l.h", "qvti_aux.h")
+
+
+def kernel_source_hash():
+    """sha256 (first 16 hex digits) of the engine's kernel and launch sources:
+    a committed counter summary is only used while it matches."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "q-vti_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(name, kernel):
     """HBM bytes per launch of `kernel` on workload `name` from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy probe), or
-    None when no summary covers it."""
+    None when no summary covers it or the kernel sources changed since."""
     path = os.path.join(ROOT, "profiles", "%s_traffic.json" % name)
     try:
         d = json.load(open(path))
+        if d.get("kernel_sources_sha16") != kernel_source_hash():
+            return None
         k = d["kernels"][kernel]
         return round(k["fetch_bytes"] + k["write_bytes"])
     except (OSError, KeyError, ValueError):
         return None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def run_cpu_baseline(name, budget_s=25.0):
@@ -172,6 +202,7 @@ def run_cpu_baseline(name, budget_s=25.0):
         per = (t3 - t1) / 2 if t3 > t1 else t3 / 3
         steps = 3
     return {"value": N / per / 1e9, "unit": "Gcell-steps/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": "%s FP32 oracle, %d-step run(s), per-step time from the difference of a 3-step and a "
                       "1-step run (setup excluded)" % (name, steps)}
 
@@ -202,43 +233,74 @@ def run_batched(steps, warmup, nshots=32, nt_full=2000, widths=(8, 16, 32), work
     """32 shots of one model (default: the 160x160x128 VTI workload): the
     plain path (one shot per launch sequence) against batched launches (W
     shots per kernel launch, W picked by timing each candidate).  shots/s is
-    for nt_full-step shots, from the timed per-step wall time of each path."""
+    for nt_full-step shots, from the timed per-step wall time of each path.
+    At the chosen width a second region launches the step's kernels directly
+    with HIP events around each launch: the sum of kernel times per step
+    against the HIP-graph wall time per step is the launch overhead a
+    persistent kernel could remove."""
     g, m, _ = make_workload(workload)
     shots = surface_shots(g, nshots, 11)
-    nt = warmup + steps
+    nt = warmup + 2 * steps
     rik = qvti.ricker(nt, g["dt"], g["f0"])
     with engine(g) as e:
         e.set_model_dict(m)
         e.set_probe(0)
         plain_wall = time_steps(e, nt, rik, shots[:1], steps, warmup)
     plain_step = plain_wall / steps
-    trials = {}
+    trials, ksum = {}, {}
     for w in widths:
+        if w > nshots:
+            continue
         with engine(g, max_shots=w) as e:
             e.set_model_dict(m)
             e.set_probe(0)
             wall = time_steps(e, nt, rik, shots[:w], steps, warmup)
-        trials[w] = wall / steps / w  # seconds per shot-step
+            trials[w] = wall / steps / w  # seconds per shot-step
+            e.enable_kernel_timing(steps)
+            e.step(steps)
+            e.sync()
+            kt = e.kernel_times()
+            ksum[w] = (sum(v["total_ms"] for v in kt.values()) / steps / 1e3, wall / steps)
     wbest = min(trials, key=trials.get)
+    kernel_sum = (wbest,) + ksum[wbest]
     per_shot_step = trials[wbest]
     N = g["nx"] * g["ny"] * g["nz"]
+    balg = b_alg(g["nx"], g["ny"], g["nz"], g["pml"], g["vti"])
     plain_sps = 1.0 / (plain_step * nt_full)
     batched_sps = 1.0 / (per_shot_step * nt_full)
+    gcs_plain, gcs_batch = N / plain_step / 1e9, N / per_shot_step / 1e9
     wl = "%dx%dx%d %s" % (g["nx"], g["ny"], g["nz"], "VTI" if g["vti"] else "isotropic")
-    return {
+    out = {
         "metric": "batched shots/s (%s, %d-step shots)" % (wl, nt_full),
-        "value": round(batched_sps, 4), "unit": "shots/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
-        "ms_per_step": round(per_shot_step * wbest * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32",
-        "data": "synthetic (seeded %s model)" % ("smooth random VTI" if g["vti"] else "homogeneous isotropic"),
+        "value": round(batched_sps, 4), "unit": "shots/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(per_shot_step * wbest * 1e3, 4),
         "config": {"workload": "%d shots on one %s model, pml %d, dx %.1f m, dt %.0e s, sources seeded on the "
                                "surface grid" % (nshots, wl, g["pml"], g["dx"], g["dt"]),
                    "cells": N, "parallelism": "1 GPU, %d shots per launch" % wbest},
+        "shots_per_launch": wbest,
         "plain_shots_per_s": round(plain_sps, 4),
         "speedup_vs_plain": round(batched_sps / plain_sps, 3),
-        "gcell_steps_per_s": {"plain": round(N / plain_step / 1e9, 3), "batched": round(N / per_shot_step / 1e9, 3)},
-        "width_trials_ms_per_shot_step": {str(w): round(v * 1e3, 4) for w, v in trials.items()},
+        "gcell_steps_per_s": {"plain": round(gcs_plain, 3), "batched": round(gcs_batch, 3)},
+        "roofline_frac": {"b_alg_bytes_per_cell_step": round(balg, 1), "plain": round(gcs_plain * balg / HBM_PEAK_GBS, 4),
+                          "batched": round(gcs_batch * balg / HBM_PEAK_GBS, 4)},
+        "width_trials_ms_per_shot_step": {str(w): round(v * 1e3, 4) for w, v in sorted(trials.items())},
     }
+    w, ks, wall = kernel_sum
+    out["graph_step"] = {"shots": w, "kernel_sum_ms": round(ks * 1e3, 4), "graph_wall_ms": round(wall * 1e3, 4),
+                         "launch_overhead_frac": round(1.0 - ks / wall, 4)}
+    return out
+
+
+def stream_copy(nbytes=2 << 30, reps=10):
+    """Measured device-to-device stream copy: hipMemcpyAsync (the HIP
+    runtime's blit kernel) of 2 GiB, HIP-event timed on one stream; bytes
+    read + written per second."""
+    import hiprt
+
+    s, bps = hiprt.stream_copy(nbytes, reps)
+    gbs = bps / 1e9
+    return {"GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "bytes_each_way": nbytes,
+            "reps": reps, "us_per_copy": round(s * 1e6, 1), "kernel": "HIP runtime device-to-device copy"}
 
 
 SERVE_GRIDS = [(96, 96, 96), (128, 128, 128), (160, 160, 128), (200, 200, 100)]
@@ -335,17 +397,29 @@ def run_served(nreq, max_active=4, slice_steps=50):
 
 
 def copy_probe(mib=1024, reps=3):
-    """A plain device copy of known size (torch's elementwise copy kernel):
-    the reference point that scales hardware FETCH_SIZE / WRITE_SIZE counters
-    to bytes."""
-    import torch
+    """A plain device copy of known size (the HIP runtime's device-to-device
+    blit kernel): the reference point that scales hardware FETCH_SIZE /
+    WRITE_SIZE counters to bytes."""
+    import hiprt
 
-    x = torch.ones(mib << 18, device="cuda")
-    y = torch.empty_like(x)
-    for _ in range(reps):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    return {"copy_probe_bytes_each_way": x.numel() * 4, "reps": reps}
+    hiprt.stream_copy(mib << 20, reps)
+    return {"copy_probe_bytes_each_way": mib << 20, "reps": reps}
+
+
+LEG_DROP = ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "dtype", "data")
+
+
+def leg(fn, *a, **k):
+    """One extra leg of the default line: its result without the keys the
+    headline already states, or the error it raised (the headline stands on
+    its own)."""
+    t0 = time.perf_counter()
+    try:
+        out = {key: v for key, v in fn(*a, **k).items() if key not in LEG_DROP}
+    except Exception as ex:
+        out = {"error": "%s: %s" % (type(ex).__name__, ex)}
+    out["leg_wall_s"] = round(time.perf_counter() - t0, 1)
+    return out
 
 
 def main():
@@ -355,6 +429,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="vti256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the batched, served and stream-copy legs of the default line")
     ap.add_argument("--batched", action="store_true", help="32-shot batched leg instead (--workload, default vti160)")
     ap.add_argument("--served", type=int, default=0, metavar="NREQ", help="serving leg with NREQ requests instead")
     ap.add_argument("--copy-probe", action="store_true", help="only run a 1 GiB device copy (counter calibration)")
@@ -430,6 +506,10 @@ def main():
     if blow_up is not None:
         out["stability"] = ("the blow-up probe fired by step %d: dt %.0e s is beyond the scheme's stability limit "
                             "for this model (the work per step is unchanged)" % (blow_up, g["dt"]))
+    if not args.headline_only and args.kspace == 0 and name == "vti256":
+        out["batched"] = leg(run_batched, 30, 5)
+        out["served"] = leg(run_served, 16)
+        out["stream_copy"] = leg(stream_copy)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = run_cpu_baseline(name)

@@ -30,8 +30,12 @@ class QoConfig(ctypes.Structure):
         ("vti", ctypes.c_int), ("snap_every", ctypes.c_int), ("snap_field", ctypes.c_int),
         ("nthreads", ctypes.c_int),
         ("ks_vref", ctypes.c_float), ("ks_fcut", ctypes.c_float), ("ks_vmax", ctypes.c_float),
-        ("ks_taper", ctypes.c_float),
+        ("ks_taper", ctypes.c_float), ("slip", ctypes.c_int),
     ]
+
+
+# planted mistakes (slip tests only): qvti_oracle.h QO_SLIP_*
+SLIP_ETA_SWAP, SLIP_NO_TAU, SLIP_SHEAR_DPLUS, SLIP_PML_NODE_V, SLIP_PML_NODE_S, SLIP_ETA_SIGN = 1, 2, 4, 8, 16, 32
 
 
 _lib = None
@@ -123,7 +127,7 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
-def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_field=0, kspace=None):
+def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_field=0, kspace=None, slip=0):
     """Run the oracle on one shot.
 
     shot: dict(nx, ny, nz, pml, nt, dx, dy, dz, dt, f0, w0, src_kind, sx, sy, sz,
@@ -131,6 +135,7 @@ def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_fie
     model: dict(vp, gp, vs, gs, rho[, eps, delta]) float32 arrays of shape (nx, ny, nz)
     kspace: optional dict(vref, fcut, vmax, taper) of the opt-in k-space
     correction / wavenumber filter (absent: the reference scheme).
+    slip: SLIP_* bits of a planted mistake (slip tests only; 0: none).
     Returns dict with records [3][nt][nrec] and optionally state [9][nx][ny][nz]
     and snapshots.
     """
@@ -147,6 +152,7 @@ def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_fie
                    ry.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                    rz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                    vti, snap_every, snap_field, nthreads)
+    cfg.slip = int(slip)
     if kspace:
         cfg.ks_vref = kspace.get("vref", 0.0)
         cfg.ks_fcut = kspace.get("fcut", 0.0)

@@ -51,7 +51,21 @@ typedef struct {
      * scheme): sinc^2(ks_vref |k| dt / 2) and a radial Tukey taper with
      * cut-off 2 pi ks_fcut / ks_vmax and taper ratio ks_taper */
     float ks_vref, ks_fcut, ks_vmax, ks_taper;
+    /* Planted mistakes (QO_SLIP_* bits; 0 = the reference scheme).  Only for
+     * tests that show a physics check fails on a plausible misreading of the
+     * reference; never set otherwise. */
+    int slip;
 } qo_config;
+
+/* Planted mistakes for the slip tests. */
+enum {
+    QO_SLIP_ETA_SWAP = 1,    /* eta1 <-> eta3: |k|^-1 and |k| terms exchanged */
+    QO_SLIP_NO_TAU = 2,      /* tau1 = tau2 = 0: the d/dt loss terms dropped */
+    QO_SLIP_SHEAR_DPLUS = 4, /* dz vx (for pxz) with D+ instead of D- */
+    QO_SLIP_PML_NODE_V = 8,  /* phi_pxx_x with node instead of half-cell a, b */
+    QO_SLIP_PML_NODE_S = 16, /* phi_vxz with node instead of half-cell a, b */
+    QO_SLIP_ETA_SIGN = 32    /* eta1, eta3 negated: the dispersion term reversed */
+};
 
 /* Full forward run.  Model arrays are FP32 over the total grid, index
  * ix*ny*nz + iy*nz + iz; gp/gs are gamma = atan(1/Q)/pi (caller computes, as

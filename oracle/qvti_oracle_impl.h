@@ -557,16 +557,26 @@ int FN(qo_run)(const qo_config *c, const float *rik,
         cH[j] = c->vti ? malloc(sizeof(REAL) * N) : NULL;
         cD[j] = c->vti ? malloc(sizeof(REAL) * N) : NULL;
     }
+    /* planted mistakes (slip tests only): applied to every coefficient set */
+#define QSLIP(q)                                                     \
+    do {                                                             \
+        if (c->slip & QO_SLIP_ETA_SWAP) { double t_ = (q)[0]; (q)[0] = (q)[2]; (q)[2] = t_; } \
+        if (c->slip & QO_SLIP_NO_TAU) { (q)[3] = 0.0; (q)[4] = 0.0; } \
+        if (c->slip & QO_SLIP_ETA_SIGN) { (q)[0] = -(q)[0]; (q)[2] = -(q)[2]; } \
+    } while (0)
 #pragma omp parallel for schedule(static)
     for (long ip = 0; ip < N; ip++) {
         double q[5];
         qo_q_coeffs(rho_in[ip], vp[ip], gp[ip], c->w0, q);
+        QSLIP(q);
         for (int j = 0; j < 5; j++) cP[j][ip] = (REAL)q[j];
         qo_q_coeffs(rho_in[ip], vs[ip], gs[ip], c->w0, q);
+        QSLIP(q);
         for (int j = 0; j < 5; j++) cS[j][ip] = (REAL)q[j];
         if (c->vti) {
             double vh = (eps[ip] == 0.0f) ? (double)vp[ip] : (double)vp[ip] * sqrt(1.0 + 2.0 * (double)eps[ip]);
             qo_q_coeffs(rho_in[ip], vh, gp[ip], c->w0, q);
+            QSLIP(q);
             for (int j = 0; j < 5; j++) cH[j][ip] = (REAL)q[j];
             double vd;
             if (del[ip] == 0.0f) {
@@ -577,9 +587,11 @@ int FN(qo_run)(const qo_config *c, const float *rik,
                 vd = sqrt((sqrt(prod > 0 ? prod : 0.0) + c44) / r);
             }
             qo_q_coeffs(rho_in[ip], vd, gp[ip], c->w0, q);
+            QSLIP(q);
             for (int j = 0; j < 5; j++) cD[j][ip] = (REAL)q[j];
         }
     }
+#undef QSLIP
 
     /* -------- C-PML profiles */
     REAL *ax = malloc(sizeof(REAL) * nx), *bx = malloc(sizeof(REAL) * nx);
@@ -654,7 +666,9 @@ int FN(qo_run)(const qo_config *c, const float *rik,
                 int ix = (int)(ip / sx), iy = (int)((ip / sy) % ny), iz = (int)(ip % nz);
                 if (ix >= nx - 1 || iy >= ny - 1 || iz >= nz - 1) continue;
                 REAL *ph;
-                ph = phi[PPXXX]; ph[ip] = ph[ip] + dt * (-0.5 * (ax[ix] + ax[ix + 1]) * ph[ip] - 0.5 * (bx[ix] + bx[ix + 1]) * px1[ip] / ntp);
+                ph = phi[PPXXX];
+                if (c->slip & QO_SLIP_PML_NODE_V) ph[ip] = ph[ip] + dt * (-ax[ix] * ph[ip] - bx[ix] * px1[ip] / ntp);
+                else ph[ip] = ph[ip] + dt * (-0.5 * (ax[ix] + ax[ix + 1]) * ph[ip] - 0.5 * (bx[ix] + bx[ix + 1]) * px1[ip] / ntp);
                 ph = phi[PPXYY]; ph[ip] = ph[ip] + dt * (-ay[iy] * ph[ip] - by[iy] * py1[ip] / ntp);
                 ph = phi[PPXZZ]; ph[ip] = ph[ip] + dt * (-az[iz] * ph[ip] - bz[iz] * pz1[ip] / ntp);
                 ph = phi[PPXYX]; ph[ip] = ph[ip] + dt * (-ax[ix] * ph[ip] - bx[ix] * px3[ip] / ntp);
@@ -689,6 +703,7 @@ int FN(qo_run)(const qo_config *c, const float *rik,
             {0, dpx, 0}, {1, dpy, 1}, {2, dpz, 2}, {0, dmz, 2}, {2, dmx, 0},
             {0, dmy, 1}, {1, dmx, 0}, {1, dmz, 2}, {2, dmy, 1},
         };
+        if (c->slip & QO_SLIP_SHEAR_DPLUS) sop[3].tab = dpz;
         for (int g = 0; g < 9; g++) {
             const REAL *S = spec[6 + sop[g].comp], *P = dv[sop[g].comp];
             for (int j = 1; j <= 5; j++) {
@@ -743,7 +758,9 @@ int FN(qo_run)(const qo_config *c, const float *rik,
                 pzz[ip] = pzz[ip] + dt * (1.0 * s13x + 1.0 * s13y + 1.0 * s33z + (phi[PVXXZ][ip] + phi[PVYYZ][ip] + phi[PVZZZ][ip]));
 
                 double sb_vxz = SBAR(3), sb_vzx = SBAR(4), sb_vxy = SBAR(5), sb_vyx = SBAR(6), sb_vyz = SBAR(7), sb_vzy = SBAR(8);
-                ph = phi[PVXZ]; ph[ip] = ph[ip] + dt * (-0.5 * (az[iz] + az[iz + 1]) * ph[ip] - 0.5 * (bz[iz] + bz[iz + 1]) * sb_vxz);
+                ph = phi[PVXZ];
+                if (c->slip & QO_SLIP_PML_NODE_S) ph[ip] = ph[ip] + dt * (-az[iz] * ph[ip] - bz[iz] * sb_vxz);
+                else ph[ip] = ph[ip] + dt * (-0.5 * (az[iz] + az[iz + 1]) * ph[ip] - 0.5 * (bz[iz] + bz[iz + 1]) * sb_vxz);
                 ph = phi[PVZX]; ph[ip] = ph[ip] + dt * (-0.5 * (ax[ix] + ax[ix + 1]) * ph[ip] - 0.5 * (bx[ix] + bx[ix + 1]) * sb_vzx);
                 ph = phi[PVXY]; ph[ip] = ph[ip] + dt * (-0.5 * (ay[iy] + ay[iy + 1]) * ph[ip] - 0.5 * (by[iy] + by[iy + 1]) * sb_vxy);
                 ph = phi[PVYX]; ph[ip] = ph[ip] + dt * (-0.5 * (ax[ix] + ax[ix + 1]) * ph[ip] - 0.5 * (bx[ix] + bx[ix + 1]) * sb_vyx);

@@ -27,10 +27,12 @@ using namespace qvti;
 #define HIPCHK(x)                                                                          \
     do {                                                                                   \
         hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
+        if (e_ != hipSuccess) {                                                            \
+            (void)hipGetLastError(); /* reported here, not again by a later check */       \
             return qvti_fail(e_ == hipErrorOutOfMemory ? QVTI_E_NOMEM : QVTI_E_DEVICE,     \
                              "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
                              __LINE__);                                                    \
+        }                                                                                  \
     } while (0)
 
 // ------------------------------------------------------------ lengths ----
@@ -236,7 +238,8 @@ struct qvti_ctx {
     float* snap_host = nullptr;
     long long snap_cap = 0;
     float* d_snap = nullptr;
-    long long d_snap_cap = 0;
+    long long d_snap_alloc = 0;  // floats allocated at d_snap (only grows)
+    long long snap_lim = 0;      // floats this run may write: min(box * snapshots, snap_cap)
     // graphs
     bool use_graphs = true;
     hipGraphExec_t gexec = nullptr;
@@ -277,6 +280,7 @@ static int dalloc(qvti_ctx* c, void** p, size_t bytes)
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
+        (void)hipGetLastError();  // the failure is reported here, not by a later launch check
         return qvti_fail(QVTI_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     }
     c->allocs.push_back(*p);
@@ -983,7 +987,7 @@ static void enqueue_step_with_snap(qvti_ctx* c)
         sa.every = c->snap_every;
         sa.it = c->d_it;
         sa.out = c->d_snap;
-        sa.cap = c->d_snap_cap;
+        sa.cap = c->snap_lim;
         hipLaunchKernelGGL(k_snapshot, dim3(256), dim3(kThreads), 0, c->stream, sa);
         if (e0 >= 0) record_event(c, -1 - KC_SNAP);
     }
@@ -1040,6 +1044,15 @@ extern "C" int qvti_begin(qvti_ctx* c, int nt, const float* rik, const qvti_shot
         }
         off[s + 1] = off[s] + sh.nrec;
     }
+    // From here on the previous run's buffers may be freed: the context is
+    // not steppable again until this call succeeds (its captured graph holds
+    // the previous run's kernel arguments).
+    c->begun = 0;
+    c->steps_done = 0;
+    if (c->gexec) {
+        hipGraphExecDestroy(c->gexec);
+        c->gexec = nullptr;
+    }
     // (re)allocate run buffers
     auto realloc_dev = [&](void** p, size_t bytes) -> int {
         if (*p) {
@@ -1082,16 +1095,20 @@ extern "C" int qvti_begin(qvti_ctx* c, int nt, const float* rik, const qvti_shot
     const int minus1 = -1, zero = 0;
     HIPCHK(hipMemcpyAsync(c->d_it, &minus1, sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->d_probe, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    // snapshots
+    // snapshots: the run writes at most min(box * snapshots, host capacity)
+    // floats, whatever an earlier run on this context allocated
+    c->snap_lim = 0;
     if (c->snap_every > 0) {
         const long long W = snap_cells(c);
         const long long nsnap = (nt + c->snap_every - 1) / c->snap_every;
         const long long need = std::min(W * nsnap, c->snap_cap);
-        if (need > c->d_snap_cap) {
+        if (need > c->d_snap_alloc) {
+            c->d_snap_alloc = 0;
             rc = realloc_dev((void**)&c->d_snap, sizeof(float) * need);
             if (rc) return rc;
-            c->d_snap_cap = need;
+            c->d_snap_alloc = need;
         }
+        c->snap_lim = need;
     }
     // per-run kernel arguments
     c->a_vz.nshots = nshots;
@@ -1197,7 +1214,7 @@ extern "C" int qvti_finish(qvti_ctx* c, float* rec_vx, float* rec_vy, float* rec
     if (c->snap_every > 0 && c->d_snap && c->snap_host) {
         const long long W = snap_cells(c);
         const long long nsnap = (c->steps_done - 1) / c->snap_every + 1;
-        const long long cnt = std::min(W * nsnap, c->d_snap_cap);
+        const long long cnt = std::min(W * nsnap, c->snap_lim);
         HIPCHK(hipMemcpy(c->snap_host, c->d_snap, sizeof(float) * cnt, hipMemcpyDeviceToHost));
     }
     return rc;
@@ -1233,7 +1250,7 @@ std::vector<std::pair<float*, long long>> ckpt_regions(qvti_ctx* c)
     if (c->snap_every > 0 && c->d_snap && c->steps_done > 0) {
         const long long W = snap_cells(c);
         const long long taken = (c->steps_done - 1) / c->snap_every + 1;
-        r.push_back({c->d_snap, std::min(W * taken, c->d_snap_cap)});
+        r.push_back({c->d_snap, std::min(W * taken, c->snap_lim)});
     }
     return r;
 }
@@ -1597,7 +1614,7 @@ extern "C" int qvti_fft3d(int nx, int ny, int nz, int sign, float* host_data)
         std::vector<float2> t = twiddles(dims[a]);
         if (hipMalloc(&tw[a], sizeof(float2) * dims[a]) != hipSuccess ||
             hipMemcpy(tw[a], t.data(), sizeof(float2) * dims[a], hipMemcpyHostToDevice) != hipSuccess)
-            rc = qvti_fail(QVTI_E_DEVICE, "table upload failed");
+            rc = qvti_fail(QVTI_E_DEVICE, "table upload failed: %s", hipGetErrorString(hipGetLastError()));
     }
     if (!rc && hipMemcpy(d, host_data, sizeof(float2) * N, hipMemcpyHostToDevice) != hipSuccess)
         rc = qvti_fail(QVTI_E_DEVICE, "H2D failed");
@@ -1649,7 +1666,7 @@ extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym,
     int rc = QVTI_OK;
     if (hipMalloc(&din, sizeof(float) * N) != hipSuccess || hipMalloc(&dout, sizeof(float) * N) != hipSuccess ||
         hipMalloc(&dtw, sizeof(float2) * n) != hipSuccess || hipMalloc(&dsym, sizeof(float2) * n) != hipSuccess)
-        rc = qvti_fail(QVTI_E_NOMEM, "allocation failed");
+        rc = qvti_fail(QVTI_E_NOMEM, "allocation failed: %s", hipGetErrorString(hipGetLastError()));
     if (!rc && (hipMemcpy(din, in, sizeof(float) * N, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(dtw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(dsym, (sym ? dm : dp).data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess))

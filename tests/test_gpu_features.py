@@ -189,3 +189,70 @@ def test_snapshot_window_reaches_into_the_cpml():
         with pytest.raises(qvti.QvtiError) as err:
             e2.set_snapshot(field, every, nt, window=((0, 0, 0), (25, 20, 32)))
         assert err.value.code == qvti.E_ARG
+
+
+def test_truncated_snapshot_run_after_a_large_one_stays_in_the_host_buffer():
+    """A context that ran with a large snapshot buffer, then runs with a
+    smaller host capacity than its snapshots need (truncation), writes at
+    most that capacity: a guard region after the host buffer stays intact,
+    the floats it writes equal the start of the full run's snapshots, and a
+    checkpoint of the truncated run restores into a fresh context."""
+    shot, model = small_case(24, 20, 32, 4, 30, O.SRC_EXPLOSIVE, True)
+    nt, every, field = shot["nt"], 5, 2
+    guard = 4096
+    with engine_for(shot, model) as e:
+        snaps_full = e.set_snapshot(field, every, nt)
+        e.begin(nt, shot["rik"], [shot_of(shot)])
+        e.step(nt)
+        e.finish()
+        full = snaps_full.copy()
+        W = full[0].size
+        cap = 2 * W + W // 3  # two whole snapshots and part of a third
+        buf = np.full(cap + guard, 12345.0, np.float32)
+        qvti.check(qvti.lib().qvti_set_snapshot(e._c, field, every, buf.ctypes.data, cap))
+        e.begin(nt, shot["rik"], [shot_of(shot)])
+        e.step(nt)
+        e.finish()
+        assert np.all(buf[cap:] == 12345.0), "snapshot copy wrote past the host capacity"
+        assert np.array_equal(buf[:2 * W], full.reshape(-1)[:2 * W])
+        # checkpoint half way through a second truncated run on the reused context
+        buf_a = np.full(cap + guard, 12345.0, np.float32)
+        qvti.check(qvti.lib().qvti_set_snapshot(e._c, field, every, buf_a.ctypes.data, cap))
+        e.begin(nt, shot["rik"], [shot_of(shot)])
+        e.step(nt // 2)
+        img = e.checkpoint()
+    with engine_for(shot, model) as f:
+        buf_b = np.full(cap + guard, 12345.0, np.float32)
+        qvti.check(qvti.lib().qvti_set_snapshot(f._c, field, every, buf_b.ctypes.data, cap))
+        f.begin(nt, shot["rik"], [shot_of(shot)])
+        f.restore(img)
+        f.step(nt - nt // 2)
+        f.finish()
+    assert np.all(buf_b[cap:] == 12345.0)
+    assert np.array_equal(buf_b[:2 * W], full.reshape(-1)[:2 * W])
+
+
+def test_step_after_a_failed_begin_reports_state_error():
+    """A qvti_begin that fails after it has released the previous run's
+    buffers (here: a receiver record too large for the device) leaves the
+    context not steppable - qvti_step returns QVTI_E_STATE instead of
+    replaying the previous run's graph - and a later valid begin works."""
+    shot, model = small_case(16, 20, 24, 3, 20, O.SRC_FX, False)
+    with engine_for(shot, model) as e:
+        rec_ok = e.forward(shot["nt"], shot["rik"], shot_of(shot))
+        e.begin(shot["nt"], shot["rik"], [shot_of(shot)])
+        e.step(5)
+        e.sync()
+        nrec = 1_000_000  # 3 * nrec * nt floats: about 1.2 TB of records
+        big = qvti.make_shot(O.SRC_FX, (shot["sx"], shot["sy"], shot["sz"]), np.zeros(nrec, np.int32),
+                             np.zeros(nrec, np.int32), np.zeros(nrec, np.int32))
+        nt_big = 100_000
+        rik_big = np.zeros(nt_big, np.float32)
+        with pytest.raises(qvti.QvtiError) as err:
+            e.begin(nt_big, rik_big, [big])
+        assert err.value.code == qvti.E_NOMEM
+        with pytest.raises(qvti.QvtiError) as err:
+            e.step(1)
+        assert err.value.code == qvti.E_STATE
+        rec = e.forward(shot["nt"], shot["rik"], shot_of(shot))
+        assert np.array_equal(rec, rec_ok)
