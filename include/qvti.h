@@ -204,6 +204,17 @@ long long qvti_estimate_bytes(const qvti_grid *g, int max_shots);
 /* ---- standalone transforms on the engine's kernels (tests / tuning).
  * In-place unnormalised 3-D C2C of interleaved complex data [nx][ny][nz]. */
 int qvti_fft3d(int nx, int ny, int nz, int sign, float *host_data);
+/* Allocation-free variant of qvti_fft3d for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same transform,
+ * but on DEVICE data d_data (interleaved complex [nx][ny][nz], caller-owned),
+ * enqueued on `stream` (a hipStream_t; NULL: the null stream) without
+ * waiting.  Its only other device memory is a static 12 KiB per-device
+ * scratch array holding the three axes' twiddle tables (each axis <= 512;
+ * longer axes are rejected with QVTI_E_UNSUPPORTED); every call rebuilds the
+ * tables it uses.  Calls that overlap on one device race on that array:
+ * serialise them (one stream, or events).  Calls no hipMalloc/hipFree. */
+int qvti_fft3d_noalloc(int nx, int ny, int nz, int sign, float *d_data, void *stream);
 /* 1-D shift-derivative of a real field along `axis` (0 x, 1 y, 2 z) with the
  * reference's D+ (sym 0) or D- (sym 1) symbol, /n normalised. */
 int qvti_deriv1d(int nx, int ny, int nz, float d, int axis, int sym, const float *in, float *out);

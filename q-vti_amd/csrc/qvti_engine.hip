@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1588,13 +1589,12 @@ extern "C" int qvti_kernel_bytes(qvti_ctx* c, int max_classes, double* bytes)
 }
 
 // ------------------------------------------------ standalone transforms ---
+// W_n^i for the standalone transforms (twiddle_w: the same bits as the
+// device-built tables of qvti_fft3d_noalloc)
 static std::vector<float2> twiddles(int n)
 {
     std::vector<float2> t(n);
-    for (int i = 0; i < n; ++i) {
-        const double ang = -2.0 * M_PI * i / n;
-        t[i] = make_float2((float)cos(ang), (float)sin(ang));
-    }
+    for (int i = 0; i < n; ++i) t[i] = twiddle_w(i, n);
     return t;
 }
 
@@ -1646,6 +1646,91 @@ extern "C" int qvti_fft3d(int nx, int ny, int nz, int sign, float* host_data)
     hipFree(d);
     for (auto p : tw) if (p) hipFree(p);
     return rc;
+}
+
+// ------------------------------------------- allocation-free transform ---
+// qvti_fft3d_noalloc keeps its only device memory beyond the caller's data -
+// the three axes' twiddle tables W_n^e, e < n <= kMaxN - in this file-scope
+// array.  Each GPU holds its own copy; host code reaches it only through the
+// per-device address from hipGetSymbolAddress, never through the symbol.
+constexpr int kNoAllocTw = 3 * kMaxN;  // float2 entries: 12 KiB per device
+static __device__ __align__(256) float2 scratch[kNoAllocTw];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float2* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per
+// device, with that device current).
+int scratch_ptr(int dev, float2** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return qvti_fail(QVTI_E_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return qvti_fail(QVTI_E_DEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float2*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return QVTI_OK;
+}
+}  // namespace
+
+// tw[a * kMaxN + e] = W_{n_a}^e for e < n_a (the three axes' tables)
+__global__ __launch_bounds__(256) void k_twiddle_tables(float2* tw, int nx, int ny, int nz)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int a = t / kMaxN, e = t - a * kMaxN;
+    const int n = a == 0 ? nx : (a == 1 ? ny : nz);
+    if (a < 3 && e < n) tw[t] = twiddle_w(e, n);
+}
+
+extern "C" int qvti_fft3d_noalloc(int nx, int ny, int nz, int sign, float* d_data, void* stream)
+{
+    if (!d_data || (sign != 1 && sign != -1)) return qvti_fail(QVTI_E_ARG, "bad arguments");
+    const int dims[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) {
+        if (dims[a] > kMaxN)
+            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d exceeds the %d-entry twiddle scratch", a,
+                             dims[a], kMaxN);
+        if (!supported_n(dims[a]))
+            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d: supported lengths are %s", a, dims[a], kSupportedList);
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    float2* tw = nullptr;
+    const int rc = scratch_ptr(dev, &tw);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    float2* d = reinterpret_cast<float2*>(d_data);
+    const long long N = (long long)nx * ny * nz;
+    // every call rebuilds the tables it uses: the array keeps the last call's
+    // contents, and calls that overlap on one device race on it
+    hipLaunchKernelGGL(k_twiddle_tables, dim3(3 * kMaxN / 256), dim3(256), 0, st, tw, nx, ny, nz);
+    const long long nrows = (long long)nx * ny;
+    ZRowsArgs z{};
+    z.nrows = nrows; z.shot_stride = N; z.tw = tw + 2 * kMaxN; z.mode = 2; z.dir = sign; z.nfields = 1;
+    z.cin[0] = d; z.cout[0] = d;
+    const int Pz = tile_pencils(nz);
+    dispatch_n(nz, [&](auto g) {
+        hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3((unsigned)((nrows + Pz - 1) / Pz), 1), dim3(kThreads), 0, st, z);
+    });
+    for (int axis = 1; axis >= 0; --axis) {
+        CStridedArgs y{};
+        const int P = tile_pencils(dims[axis]);
+        y.nz = nz; y.sx = (long long)ny * nz; y.sy = nz; y.axis = axis;
+        y.ztiles = (nz + P - 1) / P; y.shot_stride = N; y.tw = tw + axis * kMaxN; y.nfields = 1; y.f[0] = d;
+        const int other = axis ? nx : ny;
+        dispatch_n(dims[axis], [&](auto g) {
+            if (sign < 0) hipLaunchKernelGGL((k2_cfft_strided<decltype(g), -1>), dim3(other * y.ztiles, 1), dim3(kThreads), 0, st, y);
+            else hipLaunchKernelGGL((k2_cfft_strided<decltype(g), 1>), dim3(other * y.ztiles, 1), dim3(kThreads), 0, st, y);
+        });
+    }
+    HIPCHK(hipGetLastError());
+    return QVTI_OK;
 }
 
 extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym, const float* in, float* out)

@@ -30,6 +30,7 @@ constexpr double kCxPi = 3.14159265358979323846;
 
 constexpr double cx_taylor_sin(double x)
 {
+#pragma clang fp contract(off)
     double term = x, sum = x;
     for (int k = 1; k < 14; ++k) {
         term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
@@ -39,6 +40,7 @@ constexpr double cx_taylor_sin(double x)
 }
 constexpr double cx_taylor_cos(double x)
 {
+#pragma clang fp contract(off)
     double term = 1.0, sum = 1.0;
     for (int k = 1; k < 14; ++k) {
         term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
@@ -49,6 +51,7 @@ constexpr double cx_taylor_cos(double x)
 // cos and sin of 2*pi*e/n for integers e, n (exact octant reduction)
 constexpr double cx_cos2pi(long e, long n)
 {
+#pragma clang fp contract(off)
     e %= n;
     if (e < 0) e += n;
     // angle = 2 pi e / n; use 8*e/n octant
@@ -72,6 +75,14 @@ constexpr double cx_sin2pi(long e, long n)
 {
     // sin(a) = cos(a - pi/2) = cos(2 pi (e - n/4) / n); use 4e - n over 4n
     return cx_cos2pi(4 * e - n, 4 * n);
+}
+
+// W_n^e = exp(-2 pi i e / n) rounded to FP32 from the FP64 series above.
+// With contraction off the host and the device evaluate the same IEEE
+// operations, so a table built on either side has the same bits.
+__host__ __device__ inline float2 twiddle_w(long e, long n)
+{
+    return make_float2((float)cx_cos2pi(e, n), (float)(-cx_sin2pi(e, n)));
 }
 
 // exp(-2 pi i e / N) as two FP32 constants

@@ -29,6 +29,7 @@ EXPORTS = [
     "qvti_server_cancel", "qvti_server_clock", "qvti_server_destroy", "qvti_set_autotune", "qvti_tile_choice",
     "qvti_pad_model", "qvti_load_model_raw", "qvti_write_raw", "qvti_set_kspace",
     "qvti_checkpoint_bytes", "qvti_checkpoint", "qvti_restore", "qvti_set_snapshot_window",
+    "qvti_fft3d_noalloc",
 ]
 
 
@@ -90,7 +91,7 @@ def lib():
             "qvti_finish": [vp, vp, vp, vp], "qvti_get_state": [vp, i, i, vp], "qvti_steps_done": [vp],
             "qvti_enable_kernel_timing": [vp, i], "qvti_kernel_times": [vp, i, vp, i, vp, vp],
             "qvti_kernel_bytes": [vp, i, vp], "qvti_set_graphs": [vp, i], "qvti_set_probe": [vp, i, f],
-            "qvti_fft3d": [i, i, i, i, vp], "qvti_deriv1d": [i, i, i, f, i, i, vp, vp],
+            "qvti_fft3d": [i, i, i, i, vp], "qvti_fft3d_noalloc": [i, i, i, i, vp, vp], "qvti_deriv1d": [i, i, i, f, i, i, vp, vp],
             "qvti_ricker": [i, f, f, vp], "qvti_gamma_from_q": [vp, vp, ll],
             "qvti_reference_receivers": [i, i, i, i, vp, vp],
             "qvti_make_model": [i, ctypes.c_uint64, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp],
