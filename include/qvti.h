@@ -27,7 +27,7 @@ extern "C" {
 #define QVTI_E_ARG (-1)         /* invalid argument / shape */
 #define QVTI_E_DEVICE (-2)      /* HIP runtime error or no device */
 #define QVTI_E_NOMEM (-3)       /* device allocation failed */
-#define QVTI_E_UNSUPPORTED (-4) /* grid size not 2/3/5-smooth, odd, or > 512 */
+#define QVTI_E_UNSUPPORTED (-4) /* axis length not an even 2/3/5-smooth number in [16, 512] */
 #define QVTI_E_STATE (-5)       /* call out of order (e.g. no model set) */
 #define QVTI_E_UNSTABLE (-6)    /* blow-up probe fired (NaN/Inf or |v| > limit) */
 #define QVTI_E_BUDGET (-7)      /* request exceeds the server's HBM budget */
@@ -48,7 +48,7 @@ typedef struct qvti_ctx qvti_ctx;
 /* Grid and scheme constants; replaces the scalar arguments of
  * cuda_Device_malloc / cuda_forward_acoustic_3D (Myfunctions.h:77-96). */
 typedef struct {
-    int nx, ny, nz;   /* total grid (ntx, nty, ntz), each even, 2/3/5-smooth, 8..512 */
+    int nx, ny, nz;   /* total grid (ntx, nty, ntz), each even, 2/3/5-smooth, 16..512 */
     int pml;          /* C-PML thickness (>= 2, 2*pml <= each n) */
     float dx, dy, dz; /* m */
     float dt;         /* s */

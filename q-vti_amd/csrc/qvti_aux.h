@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "qvti_layout.h"
+
 namespace qvti {
 
 // Receivers: rec[it][r] = v[cell_r] for the three components (after the
@@ -94,19 +96,7 @@ __device__ __forceinline__ void q_coeffs(double rho, double v, double g, double 
     o[4] = rho * (c * c) * (g * g) * pi / w0 / dt;
 }
 
-// Blocked field groups.  A group of nf per-cell fields is stored as 64-cell
-// blocks with the fields interleaved: element (f, ip) sits at
-// ((ip >> 6) * nf + f) * 64 + (ip & 63).  A thread reaches every field of its
-// cell from one address with immediate offsets f * 256 bytes, and the fields
-// of 64 neighbouring cells form one contiguous run of nf * 256 bytes.
-// N is always a multiple of 64 (every supported axis length is a multiple of 4).
-constexpr int kBlk = 64;
-__host__ __device__ __forceinline__ long long blk_index(long long ip, int nf, int f)
-{
-    return ((ip >> 6) * nf + f) * kBlk + (ip & 63);
-}
-// coefficient groups: G1 = {P 0-4, S 5-9, S-bar 10-14}, G2 = {H 0-4, D 5-9} (VTI)
-constexpr int kG1 = 15, kG2 = 10;
+// (blocked field groups and coefficient group sizes: qvti_layout.h)
 
 struct CoefArgs {
     long long N;

@@ -18,6 +18,7 @@
 #include "qvti.h"
 #include "qvti_aux.h"
 #include "qvti_internal.h"
+#include "qvti_launch.h"
 #include "qvti_pencils.h"
 
 using namespace qvti;
@@ -39,66 +40,24 @@ using namespace qvti;
 // ------------------------------------------------------------ lengths ----
 constexpr int kMaxN = 512;
 
-// Run f(L2<N1,N2>{}) for the supported axis length n; false if unsupported.
-template <class F>
-static bool dispatch_n(int n, F&& f)
+// The launch records of every supported axis length (qvti_lengths.h); each
+// is defined in its build group of qvti_kernels.hip.
+const LenOps* qvti::len_ops(int n)
 {
-    switch (n) {
-    case 16: f(L2<4, 4>{}); return true;
-    case 20: f(L2<4, 5>{}); return true;
-    case 24: f(L2<4, 6>{}); return true;
-    case 32: f(L2<4, 8>{}); return true;
-    case 40: f(L2<5, 8>{}); return true;
-    case 48: f(L2<6, 8>{}); return true;
-    case 64: f(L2<8, 8>{}); return true;
-    case 96: f(L2<8, 12>{}); return true;
-    case 100: f(L2<10, 10>{}); return true;
-    case 128: f(L2<8, 16>{}); return true;
-    case 160: f(L2<10, 16>{}); return true;
-    case 200: f(L2<10, 20>{}); return true;
-    case 256: f(L2<16, 16>{}); return true;
-    case 400: f(L2<20, 20>{}); return true;
-    default: return false;
-    }
-}
-static const char* kSupportedList = "16 20 24 32 40 48 64 96 100 128 160 200 256 400";
-
-// Tile widths of the strided-axis kernels: PW = 1, 2, 4 (256, 512, 1024
-// threads; P = 256*PW/NMAX pencils).  Wider tiles read longer contiguous
-// z-runs per row; they exist for n >= 64 and while the tile fits in LDS.
-template <int A, int B, class F>
-static void call_pw(int pw, F&& f)
-{
-    if constexpr (A * B >= 64 && L2<A, B, 4>::kFits) {
-        if (pw >= 4) { f(L2<A, B, 4>{}); return; }
-    }
-    if constexpr (A * B >= 64 && L2<A, B, 2>::kFits) {
-        if (pw >= 2) { f(L2<A, B, 2>{}); return; }
-    }
-    f(L2<A, B, 1>{});
+#define QVTI_LEN_PTR(n_, a, b, g) &kLenOps_##n_,
+    static const LenOps* const all[] = {QVTI_LENGTHS(QVTI_LEN_PTR)};
+#undef QVTI_LEN_PTR
+    for (const LenOps* o : all)
+        if (o->n == n) return o;
+    return nullptr;
 }
 
-// Run f(L2<N1,N2,PW'>{}) for length n with the widest supported PW' <= pw.
-template <class F>
-static bool dispatch_nw(int n, int pw, F&& f)
+const char* qvti::supported_lengths()
 {
-    switch (n) {
-    case 16: call_pw<4, 4>(pw, f); return true;
-    case 20: call_pw<4, 5>(pw, f); return true;
-    case 24: call_pw<4, 6>(pw, f); return true;
-    case 32: call_pw<4, 8>(pw, f); return true;
-    case 40: call_pw<5, 8>(pw, f); return true;
-    case 48: call_pw<6, 8>(pw, f); return true;
-    case 64: call_pw<8, 8>(pw, f); return true;
-    case 96: call_pw<8, 12>(pw, f); return true;
-    case 100: call_pw<10, 10>(pw, f); return true;
-    case 128: call_pw<8, 16>(pw, f); return true;
-    case 160: call_pw<10, 16>(pw, f); return true;
-    case 200: call_pw<10, 20>(pw, f); return true;
-    case 256: call_pw<16, 16>(pw, f); return true;
-    case 400: call_pw<20, 20>(pw, f); return true;
-    default: return false;
-    }
+#define QVTI_LEN_STR(n_, a, b, g) " " #n_
+    static const char* s = QVTI_LENGTHS(QVTI_LEN_STR) + 1;
+#undef QVTI_LEN_STR
+    return s;
 }
 
 // Effective (PW, pencils per tile, threads) of length n at requested width pw.
@@ -107,22 +66,19 @@ struct TileW {
 };
 static TileW tile_w(int n, int pw)
 {
-    TileW t{0, 0, 0};
-    dispatch_nw(n, pw, [&](auto g) {
-        using G = decltype(g);
-        t = TileW{G::PW, G::P, G::T};
-    });
-    return t;
+    const LenOps* o = len_ops(n);
+    if (!o) return TileW{0, 0, 0};
+    const int i = pw_index(*o, pw);
+    return TileW{1 << i, o->P[i], o->T[i]};
 }
 
 static int tile_pencils(int n)
 {
-    int P = 0;
-    dispatch_n(n, [&](auto g) { P = decltype(g)::P; });
-    return P;
+    const LenOps* o = len_ops(n);
+    return o ? o->P[0] : 0;
 }
 
-static bool supported_n(int n) { return tile_pencils(n) > 0; }
+static bool supported_n(int n) { return len_ops(n) != nullptr; }
 
 // The reference's truncated pi (cuviscoelas_3D.cu:15).
 static const double kRefPi = 3.1415926;
@@ -208,8 +164,10 @@ struct qvti_ctx {
     hipStream_t stream = nullptr;
     qvti_grid g{};
     long long N = 0, nrows = 0;
+    long long Nb = 0;  // blk_len(N): per-shot field length of the blocked groups
     int S = 1;  // max shots
     AxisDev ax[3];
+    const LenOps* ops[3] = {nullptr, nullptr, nullptr};  // kernels of the x, y, z lengths
     // model
     float* coef = nullptr;  // blocked coefficient groups G1 (15 fields) then G2 (10, VTI)
     float* buoy = nullptr;  // [3][N]
@@ -352,7 +310,7 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
     for (int a = 0; a < 3; ++a)
         if (!supported_n(dims[a]))
             return qvti_fail(QVTI_E_UNSUPPORTED,
-                             "axis %d length %d: supported axis lengths are %s", a, dims[a], kSupportedList);
+                             "axis %d length %d: supported axis lengths are %s", a, dims[a], supported_lengths());
     if (g->pml < 2 || 2 * g->pml > g->nx || 2 * g->pml > g->ny || 2 * g->pml > g->nz)
         return qvti_fail(QVTI_E_ARG, "pml %d invalid for grid %dx%dx%d", g->pml, g->nx, g->ny, g->nz);
     if (!(g->dx > 0 && g->dy > 0 && g->dz > 0 && g->dt > 0 && g->w0 > 0 && g->f0 > 0))
@@ -367,6 +325,8 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
     c->g = *g;
     c->S = max_shots;
     c->N = (long long)g->nx * g->ny * g->nz;
+    c->Nb = blk_len(c->N);
+    for (int a = 0; a < 3; ++a) c->ops[a] = len_ops(dims[a]);
     for (int a = 0; a < 3; ++a) {
         c->snap_lo[a] = g->pml;
         c->snap_hi[a] = dims[a] - g->pml;
@@ -404,14 +364,15 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
     c->slab[0] = (long long)p2 * g->ny * g->nz;
     c->slab[1] = (long long)g->nx * p2 * g->nz;
     c->slab[2] = (long long)g->nx * g->ny * p2;
-    rc = rc ? rc : dalloc(c, (void**)&c->coef, sizeof(float) * (kG1 + (g->vti ? kG2 : 0)) * N);
+    const long long Nb = c->Nb;
+    rc = rc ? rc : dalloc(c, (void**)&c->coef, sizeof(float) * (kG1 + (g->vti ? kG2 : 0)) * Nb);
     rc = rc ? rc : dalloc(c, (void**)&c->buoy, sizeof(float) * 3 * N);
     rc = rc ? rc : dalloc(c, (void**)&c->v, sizeof(float) * 3 * S * N);
     rc = rc ? rc : dalloc(c, (void**)&c->sig, sizeof(float) * 6 * S * N);
     rc = rc ? rc : dalloc(c, (void**)&c->dv, sizeof(float) * 3 * S * N);
     rc = rc ? rc : dalloc(c, (void**)&c->H, sizeof(float) * 9 * S * N);
-    rc = rc ? rc : dalloc(c, (void**)&c->bufA, sizeof(float) * 15 * S * N);
-    rc = rc ? rc : dalloc(c, (void**)&c->bufB, sizeof(float) * 15 * S * N);
+    rc = rc ? rc : dalloc(c, (void**)&c->bufA, sizeof(float) * 15 * S * Nb);
+    rc = rc ? rc : dalloc(c, (void**)&c->bufB, sizeof(float) * 15 * S * Nb);
     for (int a = 0; a < 3 && !rc; ++a) {
         rc = dalloc(c, (void**)&c->phiv[a], sizeof(float) * 3 * S * c->slab[a]);
         rc = rc ? rc : dalloc(c, (void**)&c->phis[a], sizeof(float) * c->nphis[a] * S * c->slab[a]);
@@ -643,7 +604,7 @@ static int build_launch_config(qvti_ctx* c)
     // x-derivatives: three blocked groups of 5 (ex, gx, hx), each [shot][N/64][5][64];
     // y-derivatives: one blocked group of 15
     for (int k = 0; k < 15; ++k) {
-        DX[k] = c->bufA + (long long)(k / 5) * 5 * c->S * N + (long long)(k % 5) * kBlk;
+        DX[k] = c->bufA + (long long)(k / 5) * 5 * c->S * c->Nb + (long long)(k % 5) * kBlk;
         DY[k] = c->bufB + (long long)k * kBlk;
     }
     StridedArgs& sxa = c->a_sx;
@@ -651,6 +612,7 @@ static int build_launch_config(qvti_ctx* c)
     sxa.mode = MODE_STORE;
     sxa.njobs = 15;
     sxa.bnf = 5;
+    sxa.blk_stride = c->Nb;
     for (int j = 0; j < 5; ++j) {
         sxa.job[j] = job(fam[0][j], DX[j], nullptr, nullptr, 0, 0);          // ex: dx vx-family, D+
         sxa.job[5 + j] = job(fam[2][j], DX[5 + j], nullptr, nullptr, 1, 0);   // gx: dx vz-family, D-
@@ -661,6 +623,7 @@ static int build_launch_config(qvti_ctx* c)
     sya.mode = MODE_STORE;
     sya.njobs = 15;
     sya.bnf = kG1;
+    sya.blk_stride = c->Nb;
     for (int j = 0; j < 5; ++j) {
         sya.job[j] = job(fam[1][j], DY[j], nullptr, nullptr, 0, 0);          // ey: dy vy-family, D+
         sya.job[5 + j] = job(fam[2][j], DY[5 + j], nullptr, nullptr, 1, 0);   // gy: dy vz-family, D-
@@ -681,10 +644,11 @@ static int build_launch_config(qvti_ctx* c)
     for (int k = 0; k < 3; ++k)
         for (int j = 0; j < 5; ++j) sz.fam[k][j] = fam[k][j];
     sz.dxb = c->bufA;
-    sz.dx_group = 5LL * c->S * N;
+    sz.dx_group = 5LL * c->S * c->Nb;
+    sz.blk_stride = c->Nb;
     sz.dyb = c->bufB;
     sz.g1 = c->coef;
-    sz.g2 = g.vti ? c->coef + (long long)kG1 * N : nullptr;
+    sz.g2 = g.vti ? c->coef + (long long)kG1 * c->Nb : nullptr;
     for (int k = 0; k < 6; ++k) sz.sig[k] = sg[k];
     for (int k = 0; k < 3; ++k) sz.vel[k] = vv[k];
     for (int k = 0; k < 5; ++k) {
@@ -742,7 +706,7 @@ extern "C" int qvti_set_model(qvti_ctx* c, const float* velp, const float* gama_
         ca.eps = c->g.vti ? tmp + 5 * N : nullptr;
         ca.del = c->g.vti ? tmp + 6 * N : nullptr;
         ca.g1 = c->coef;
-        ca.g2 = c->g.vti ? c->coef + (long long)kG1 * N : nullptr;
+        ca.g2 = c->g.vti ? c->coef + (long long)kG1 * c->Nb : nullptr;
         const unsigned nb = (unsigned)std::min<long long>((N + kThreads - 1) / kThreads, 65535LL * 4);
         hipLaunchKernelGGL(k_coefficients, dim3(nb), dim3(kThreads), 0, c->stream, ca);
         AvgArgs aa;
@@ -786,73 +750,44 @@ static unsigned xcd_grid(int ntiles, unsigned S)
 static void launch_class(qvti_ctx* c, int cls)
 {
     hipStream_t st = c->stream;
-    const dim3 blk(kThreads);
     const unsigned S = (unsigned)c->nshots;
-    const int nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
+    const LenOps &X = *c->ops[0], &Y = *c->ops[1], &Z = *c->ops[2];
+    auto strided = [&](const dim3& g) { return dim3(g.x, S, (unsigned)c->jz[cls]); };
     switch (cls) {
     case KC_VEL_X:
-        dispatch_nw(nx, c->pw[cls], [&](auto g) {
-            using G = decltype(g);
-            hipLaunchKernelGGL((k2_deriv_strided<G, MODE_VEL_FIRST>), dim3(c->g_vx.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_vx);
-        });
+        X.deriv[MODE_VEL_FIRST][pw_index(X, c->pw[cls])](strided(c->g_vx), st, c->a_vx);
         break;
     case KC_VEL_Y:
-        dispatch_nw(ny, c->pw[cls], [&](auto g) {
-            using G = decltype(g);
-            hipLaunchKernelGGL((k2_deriv_strided<G, MODE_VEL_ACC>), dim3(c->g_vy.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_vy);
-        });
+        Y.deriv[MODE_VEL_ACC][pw_index(Y, c->pw[cls])](strided(c->g_vy), st, c->a_vy);
         break;
     case KC_VEL_Z:
-        dispatch_n(nz, [&](auto g) {
-            hipLaunchKernelGGL(k2_vel_z<decltype(g)>, dim3(xcd_grid(c->a_vz.ntiles, S)), blk, 0, st, c->a_vz);
-        });
+        Z.vel_z(dim3(xcd_grid(c->a_vz.ntiles, S)), st, c->a_vz);
         break;
     case KC_Y_FWD:
-        dispatch_nw(ny, c->pw[cls], [&](auto g) {
-            using G = decltype(g);
-            hipLaunchKernelGGL((k2_cfft_strided<G, -1>), dim3(c->g_cy.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_yf);
-        });
+        Y.cfft[0][pw_index(Y, c->pw[cls])](strided(c->g_cy), st, c->a_yf);
         break;
     case KC_XPASS:
-        dispatch_nw(nx, c->pw[cls], [&](auto g) {
-            using G = decltype(g);
-            if (c->ks) hipLaunchKernelGGL((k2_xpass<G, true>), dim3(c->g_xp.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_xp);
-            else hipLaunchKernelGGL(k2_xpass<G>, dim3(c->g_xp.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_xp);
-        });
+        X.xpass[c->ks ? 1 : 0][pw_index(X, c->pw[cls])](strided(c->g_xp), st, c->a_xp);
         break;
     case KC_Y_INV:
-        dispatch_nw(ny, c->pw[cls], [&](auto g) {
-            using G = decltype(g);
-            hipLaunchKernelGGL((k2_cfft_strided<G, 1>), dim3(c->g_yi.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_yi);
-        });
+        Y.cfft[1][pw_index(Y, c->pw[cls])](strided(c->g_yi), st, c->a_yi);
         break;
-    case KC_Z_INV:
-        dispatch_n(nz, [&](auto g) {
-            const unsigned zs = QVTI_ZINV_SPLIT == 1 ? 1u : (unsigned)c->a_zi.nfields;
-            hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3(c->g_zr.x, S, zs), blk, 0, st, c->a_zi);
-        });
+    case KC_Z_INV: {
+        const unsigned zs = QVTI_ZINV_SPLIT == 1 ? 1u : (unsigned)c->a_zi.nfields;
+        Z.zrows(dim3(c->g_zr.x, S, zs), st, c->a_zi);
         break;
+    }
     case KC_STR_X:
-        dispatch_nw(nx, c->pw[cls], [&](auto g) {
-            using G = decltype(g);
-            hipLaunchKernelGGL((k2_deriv_strided<G, MODE_STORE>), dim3(c->g_sx.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_sx);
-        });
+        X.deriv[MODE_STORE][pw_index(X, c->pw[cls])](strided(c->g_sx), st, c->a_sx);
         break;
     case KC_STR_Y:
-        dispatch_nw(ny, c->pw[cls], [&](auto g) {
-            using G = decltype(g);
-            hipLaunchKernelGGL((k2_deriv_strided<G, MODE_STORE>), dim3(c->g_sy.x, S, c->jz[cls]), dim3(G::T), 0, st, c->a_sy);
-        });
+        Y.deriv[MODE_STORE][pw_index(Y, c->pw[cls])](strided(c->g_sy), st, c->a_sy);
         break;
     case KC_STR_Z:
-        dispatch_n(nz, [&](auto g) {
-            const dim3 grid(xcd_grid(c->a_sz.ntiles, S));
-            if (c->g.vti) hipLaunchKernelGGL((k2_stress_z<decltype(g), true>), grid, blk, 0, st, c->a_sz);
-            else hipLaunchKernelGGL((k2_stress_z<decltype(g), false>), grid, blk, 0, st, c->a_sz);
-        });
+        Z.stress_z[c->g.vti ? 1 : 0](dim3(xcd_grid(c->a_sz.ntiles, S)), st, c->a_sz);
         break;
     case KC_RECORD:
-        hipLaunchKernelGGL(k_record, dim3(c->g_rec.x, S), blk, 0, st, c->a_rec);
+        hipLaunchKernelGGL(k_record, dim3(c->g_rec.x, S), dim3(kThreads), 0, st, c->a_rec);
         break;
     default:
         break;
@@ -1598,13 +1533,34 @@ static std::vector<float2> twiddles(int n)
     return t;
 }
 
+// In-place unnormalised 3-D C2C of d [nx][ny][nz] on stream st: z rows,
+// then the y and x axes as strided pencils; tw[a] holds W_{n_a}^e.
+static void enqueue_fft3d(int nx, int ny, int nz, int sign, float2* d, const float2* const tw[3], hipStream_t st)
+{
+    const int dims[3] = {nx, ny, nz};
+    const long long N = (long long)nx * ny * nz, nrows = (long long)nx * ny;
+    const LenOps& Z = *len_ops(nz);
+    ZRowsArgs z{};
+    z.nrows = nrows; z.shot_stride = N; z.tw = tw[2]; z.mode = 2; z.dir = sign; z.nfields = 1;
+    z.cin[0] = d; z.cout[0] = d;
+    Z.zrows(dim3((unsigned)((nrows + Z.P[0] - 1) / Z.P[0]), 1), st, z);
+    for (int axis = 1; axis >= 0; --axis) {
+        const LenOps& A = *len_ops(dims[axis]);
+        CStridedArgs y{};
+        y.nz = nz; y.sx = (long long)ny * nz; y.sy = nz; y.axis = axis;
+        y.ztiles = (nz + A.P[0] - 1) / A.P[0]; y.shot_stride = N; y.tw = tw[axis]; y.nfields = 1; y.f[0] = d;
+        const int other = axis ? nx : ny;
+        A.cfft[sign < 0 ? 0 : 1][0](dim3((unsigned)(other * y.ztiles), 1), st, y);
+    }
+}
+
 extern "C" int qvti_fft3d(int nx, int ny, int nz, int sign, float* host_data)
 {
     if (!host_data || (sign != 1 && sign != -1)) return qvti_fail(QVTI_E_ARG, "bad arguments");
     const int dims[3] = {nx, ny, nz};
     for (int a = 0; a < 3; ++a)
         if (!supported_n(dims[a]))
-            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d: supported lengths are %s", a, dims[a], kSupportedList);
+            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d: supported lengths are %s", a, dims[a], supported_lengths());
     const long long N = (long long)nx * ny * nz;
     float2* d = nullptr;
     float2* tw[3] = {nullptr, nullptr, nullptr};
@@ -1619,25 +1575,8 @@ extern "C" int qvti_fft3d(int nx, int ny, int nz, int sign, float* host_data)
     if (!rc && hipMemcpy(d, host_data, sizeof(float2) * N, hipMemcpyHostToDevice) != hipSuccess)
         rc = qvti_fail(QVTI_E_DEVICE, "H2D failed");
     if (!rc) {
-        const long long nrows = (long long)nx * ny;
-        ZRowsArgs z{};
-        z.nrows = nrows; z.shot_stride = N; z.tw = tw[2]; z.mode = 2; z.dir = sign; z.nfields = 1;
-        z.cin[0] = d; z.cout[0] = d;
-        const int Pz = tile_pencils(nz);
-        dispatch_n(nz, [&](auto g) {
-            hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3((unsigned)((nrows + Pz - 1) / Pz), 1), dim3(kThreads), 0, 0, z);
-        });
-        for (int axis = 1; axis >= 0; --axis) {
-            CStridedArgs y{};
-            const int P = tile_pencils(dims[axis]);
-            y.nz = nz; y.sx = (long long)ny * nz; y.sy = nz; y.axis = axis;
-            y.ztiles = (nz + P - 1) / P; y.shot_stride = N; y.tw = tw[axis]; y.nfields = 1; y.f[0] = d;
-            const int other = axis ? nx : ny;
-            dispatch_n(dims[axis], [&](auto g) {
-                if (sign < 0) hipLaunchKernelGGL((k2_cfft_strided<decltype(g), -1>), dim3(other * y.ztiles, 1), dim3(kThreads), 0, 0, y);
-                else hipLaunchKernelGGL((k2_cfft_strided<decltype(g), 1>), dim3(other * y.ztiles, 1), dim3(kThreads), 0, 0, y);
-            });
-        }
+        const float2* const tws[3] = {tw[0], tw[1], tw[2]};
+        enqueue_fft3d(nx, ny, nz, sign, d, tws, nullptr);
         if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
             rc = qvti_fail(QVTI_E_DEVICE, "fft kernels failed");
     }
@@ -1697,7 +1636,7 @@ extern "C" int qvti_fft3d_noalloc(int nx, int ny, int nz, int sign, float* d_dat
             return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d exceeds the %d-entry twiddle scratch", a,
                              dims[a], kMaxN);
         if (!supported_n(dims[a]))
-            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d: supported lengths are %s", a, dims[a], kSupportedList);
+            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d: supported lengths are %s", a, dims[a], supported_lengths());
     }
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -1706,29 +1645,11 @@ extern "C" int qvti_fft3d_noalloc(int nx, int ny, int nz, int sign, float* d_dat
     if (rc) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     float2* d = reinterpret_cast<float2*>(d_data);
-    const long long N = (long long)nx * ny * nz;
     // every call rebuilds the tables it uses: the array keeps the last call's
     // contents, and calls that overlap on one device race on it
     hipLaunchKernelGGL(k_twiddle_tables, dim3(3 * kMaxN / 256), dim3(256), 0, st, tw, nx, ny, nz);
-    const long long nrows = (long long)nx * ny;
-    ZRowsArgs z{};
-    z.nrows = nrows; z.shot_stride = N; z.tw = tw + 2 * kMaxN; z.mode = 2; z.dir = sign; z.nfields = 1;
-    z.cin[0] = d; z.cout[0] = d;
-    const int Pz = tile_pencils(nz);
-    dispatch_n(nz, [&](auto g) {
-        hipLaunchKernelGGL(k2_zrows<decltype(g)>, dim3((unsigned)((nrows + Pz - 1) / Pz), 1), dim3(kThreads), 0, st, z);
-    });
-    for (int axis = 1; axis >= 0; --axis) {
-        CStridedArgs y{};
-        const int P = tile_pencils(dims[axis]);
-        y.nz = nz; y.sx = (long long)ny * nz; y.sy = nz; y.axis = axis;
-        y.ztiles = (nz + P - 1) / P; y.shot_stride = N; y.tw = tw + axis * kMaxN; y.nfields = 1; y.f[0] = d;
-        const int other = axis ? nx : ny;
-        dispatch_n(dims[axis], [&](auto g) {
-            if (sign < 0) hipLaunchKernelGGL((k2_cfft_strided<decltype(g), -1>), dim3(other * y.ztiles, 1), dim3(kThreads), 0, st, y);
-            else hipLaunchKernelGGL((k2_cfft_strided<decltype(g), 1>), dim3(other * y.ztiles, 1), dim3(kThreads), 0, st, y);
-        });
-    }
+    const float2* const tws[3] = {tw, tw + kMaxN, tw + 2 * kMaxN};
+    enqueue_fft3d(nx, ny, nz, sign, d, tws, st);
     HIPCHK(hipGetLastError());
     return QVTI_OK;
 }
@@ -1740,7 +1661,7 @@ extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym,
     const int dims[3] = {nx, ny, nz};
     for (int a = 0; a < 3; ++a)
         if (!supported_n(dims[a]))
-            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d: supported lengths are %s", a, dims[a], kSupportedList);
+            return qvti_fail(QVTI_E_UNSUPPORTED, "axis %d length %d: supported lengths are %s", a, dims[a], supported_lengths());
     const int n = dims[axis];
     std::vector<float2> tw = twiddles(n), dp, dm;
     std::vector<float> kf;
@@ -1757,7 +1678,8 @@ extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym,
                 hipMemcpy(dsym, (sym ? dm : dp).data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess))
         rc = qvti_fail(QVTI_E_DEVICE, "H2D failed");
     if (!rc) {
-        const int P = tile_pencils(n);
+        const LenOps& A = *len_ops(n);
+        const int P = A.P[0];
         if (axis < 2) {
             StridedArgs a{};
             a.nx = nx; a.ny = ny; a.nz = nz; a.pml = 0;
@@ -1769,16 +1691,12 @@ extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym,
             a.job[0].in = din; a.job[0].out = dout; a.job[0].acc = nullptr; a.job[0].phi = nullptr;
             a.job[0].sym = 0; a.job[0].half = 0;
             const int other = axis == 0 ? ny : nx;
-            dispatch_n(n, [&](auto g) {
-                hipLaunchKernelGGL((k2_deriv_strided<decltype(g), MODE_STORE>), dim3(other * a.ztiles, 1), dim3(kThreads), 0, 0, a);
-            });
+            A.deriv[MODE_STORE][0](dim3((unsigned)(other * a.ztiles), 1), nullptr, a);
         } else {
             RowDerivArgs r{};
             const long long nrows = (long long)nx * ny;
             r.nrows = nrows; r.tw = dtw; r.sym = dsym; r.in = din; r.out = dout;
-            dispatch_n(n, [&](auto g) {
-                hipLaunchKernelGGL(k2_deriv_rows<decltype(g)>, dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), dim3(kThreads), 0, 0, r);
-            });
+            A.deriv_rows(dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), nullptr, r);
         }
         if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
             rc = qvti_fail(QVTI_E_DEVICE, "derivative kernel failed");

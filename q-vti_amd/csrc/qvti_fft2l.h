@@ -214,34 +214,6 @@ __host__ __device__ __forceinline__ void rdft(float2* v)
     }
 }
 
-// Split n into N1 x N2 for the supported lengths.
-constexpr int kSupportedN[] = {16, 20, 24, 32, 40, 48, 64, 80, 96, 100, 128, 160, 192, 200, 256, 320, 384, 400, 512};
-constexpr int split_n1(int n)
-{
-    switch (n) {
-    case 16: return 4;
-    case 20: return 4;
-    case 24: return 4;
-    case 32: return 4;
-    case 40: return 5;
-    case 48: return 6;
-    case 64: return 8;
-    case 80: return 8;
-    case 96: return 8;
-    case 100: return 10;
-    case 128: return 8;
-    case 160: return 10;
-    case 192: return 12;
-    case 200: return 10;
-    case 256: return 16;
-    case 320: return 16;
-    case 384: return 16;
-    case 400: return 20;
-    case 512: return 16;
-    default: return 0;
-    }
-}
-
 // Per-length constants of the two-level scheme.  PW widens the tile of the
 // strided-axis kernels: a workgroup of T = 256*PW threads holds P = T/NMAX
 // pencils, so each row of the tile is P (complex) or 2P (real) contiguous

@@ -31,6 +31,7 @@
 //    swamp the smaller one.
 #pragma once
 #include "qvti_fft2l.h"
+#include "qvti_layout.h"
 
 namespace qvti {
 
@@ -127,6 +128,9 @@ struct StridedArgs {
     // blocked group of nf fields per shot ([shot][N/64][nf][64], see
     // blk_index), each job's out pointing at its field (group base + f*64)
     int bnf;
+    // per-shot length of a blocked group's field: N rounded up to whole
+    // 64-cell blocks (shot s of an nf-field group starts at s * nf * blk_stride)
+    long long blk_stride;
     // step counter advanced by this launch (the first kernel of a step), or null
     int* tick;
     DJob job[15];
@@ -191,7 +195,7 @@ void k2_deriv_strided(StridedArgs a)
                 inv_lb<G>(d, pen, jj);
                 if (a.bnf) {
                     // blocked group: (ip, ip+1) share a 64-cell block (ip even)
-                    float* ob = J.out + (long long)shot * a.bnf * a.shot_stride;
+                    float* ob = J.out + (long long)shot * a.bnf * a.blk_stride;
                     const unsigned ip0 = (unsigned)(poff + jj * astride + zc);
                     const unsigned bs = (unsigned)(kBlk * a.bnf);
 #pragma unroll
@@ -848,6 +852,7 @@ struct StressZArgs {
     //   dyb: ey 0-4 (dy vy-family), gy 5-9 (dy vz-family), hy 10-14 (dy vx-family)
     const float *dxb, *dyb;
     long long dx_group;
+    long long blk_stride;  // per-shot field length of the blocked groups (N rounded up to 64)
     // coefficient groups G1 {P, S, S-bar} and G2 {H, D} (VTI; else null)
     const float *g1, *g2;
     float* sig[6];  // pxx pyy pzz pxy pxz pyz
@@ -949,8 +954,8 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
     // uniform block bases; per-cell offsets below are unsigned 32-bit
     const long long cbase = r0 * n;          // flat index of the tile's first cell
     const long long fbase = soff + cbase;    // per-shot natural fields
-    const float* __restrict__ dxs = a.dxb + (long long)shot * 5 * a.shot_stride;
-    const float* __restrict__ dys = a.dyb + (long long)shot * kG1 * a.shot_stride;
+    const float* __restrict__ dxs = a.dxb + (long long)shot * 5 * a.blk_stride;
+    const float* __restrict__ dys = a.dyb + (long long)shot * kG1 * a.blk_stride;
     constexpr bool vti = VTI;  // G2 (H, D sets) present: a compile-time choice keeps the loads branch-free
     const long long sxb = (long long)shot * a.slabx_stride, syb = (long long)shot * a.slaby_stride,
                     szb = (long long)shot * a.slabz_stride;

@@ -40,13 +40,29 @@ def test_no_device_fails_loudly():
     assert b"device" in qvti.lib().qvti_last_error().lower()
 
 
+def _smooth_even(n):
+    if n % 2:
+        return False
+    for p in (2, 3, 5):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
 def test_unsupported_lengths_are_rejected_before_the_device():
+    """Axis lengths outside the even 2/3/5-smooth numbers of [16, 512] (a
+    factor 7 or 97, odd, too short, too long) are refused before any device
+    work, and the message lists exactly the even 2/3/5-smooth lengths of
+    [16, 512] as the supported ones (cuFFT plans any smooth size for the
+    reference, cuviscoelas_3D.cu:1703-1704)."""
     c = ctypes.c_void_p()
-    for dims in ((18, 16, 16), (16, 16, 49), (535, 485, 48)):
+    for dims in ((14, 16, 16), (16, 16, 49), (535, 485, 48), (16, 10, 16), (16, 16, 1024), (194, 16, 16)):
         g = qvti.Grid(*dims, 3, 10, 10, 10, 1e-3, 25, 157, 0)
         rc = qvti.lib().qvti_create(ctypes.byref(c), ctypes.byref(g), 1, 0)
         assert rc == qvti.E_UNSUPPORTED, dims
-        assert b"supported" in qvti.lib().qvti_last_error()
+        msg = qvti.lib().qvti_last_error().decode()
+        listed = [int(t) for t in msg.split("supported axis lengths are")[1].split()]
+        assert listed == [n for n in range(16, 513) if _smooth_even(n)]
 
 
 def test_ricker_matches_reference_formula():

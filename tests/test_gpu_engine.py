@@ -33,7 +33,31 @@ def shot_of(shot, sx=None):
                           shot["rx"], shot["ry"], shot["rz"])
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (20, 24, 32), (40, 48, 64), (64, 64, 64), (96, 100, 16), (128, 16, 20), (256, 16, 24), (16, 400, 20), (200, 20, 160)])
+def _smooth_even_lengths():
+    out = []
+    for n in range(16, 513, 2):
+        m = n
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            out.append(n)
+    return out
+
+
+def _every_length_shapes():
+    """Every supported axis length once, on the z, y and x axis in turn, next
+    to short lengths on the other two axes."""
+    small = [16, 18, 20, 24, 30]
+    shapes = []
+    for i, n in enumerate(_smooth_even_lengths()):
+        a, b = small[i % 5], small[(i + 2) % 5]
+        shapes.append([(a, b, n), (a, n, b), (n, a, b)][i % 3])
+    return shapes
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (20, 24, 32), (40, 48, 64), (64, 64, 64), (96, 100, 16), (128, 16, 20),
+                                   (256, 16, 24), (16, 400, 20), (200, 20, 160), (90, 54, 150)] + _every_length_shapes())
 def test_fft3d_matches_numpy(shape):
     rng = np.random.default_rng(0)
     x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
@@ -44,7 +68,8 @@ def test_fft3d_matches_numpy(shape):
         assert err < 2e-6, (shape, sign, err)
 
 
-@pytest.mark.parametrize("shape", [(16, 20, 24), (32, 16, 40), (24, 48, 20), (100, 16, 32), (16, 256, 20)])
+@pytest.mark.parametrize("shape", [(16, 20, 24), (32, 16, 40), (24, 48, 20), (100, 16, 32), (16, 256, 20), (18, 30, 54),
+                                   (90, 50, 36), (486, 18, 20), (16, 250, 30)])
 def test_deriv1d_matches_spectral_reference(shape):
     rng = np.random.default_rng(1)
     f = rng.standard_normal(shape).astype(np.float32)
@@ -72,6 +97,15 @@ CASES = [
     # the longest supported axis (400 = 20 x 20) on the row axis and on a strided axis
     (16, 20, 400, 4, 12, O.SRC_EXPLOSIVE, True),
     (400, 16, 20, 4, 12, O.SRC_FZ, False),
+    # lengths that are not multiples of 4 (grids whose cell count is not a
+    # multiple of 64) and odd radix factors 9, 15, 25, 27
+    (18, 30, 36, 4, 30, O.SRC_FX, True),
+    (50, 54, 18, 5, 25, O.SRC_EXPLOSIVE, False),
+    (90, 16, 30, 4, 20, O.SRC_FZ, True),
+    (16, 18, 486, 4, 10, O.SRC_EXPLOSIVE, True),
+    (250, 18, 16, 4, 10, O.SRC_FY, False),
+    (30, 512, 16, 4, 10, O.SRC_FZ, True),
+    (72, 150, 270, 6, 8, O.SRC_EXPLOSIVE, True),
 ]
 
 
@@ -128,12 +162,13 @@ def test_batched_shots_equal_single_shots_bitwise():
             assert np.array_equal(single, b)
 
 
-@pytest.mark.parametrize("mode", [0, 2, 4, 16, 20, 1])
-def test_tile_widths_agree_bitwise_and_match_the_oracle(mode):
+@pytest.mark.parametrize("mode,dims", [(m, (64, 96, 40)) for m in (0, 2, 4, 16, 20, 1)] +
+                         [(m, (90, 144, 54)) for m in (0, 4, 20)])
+def test_tile_widths_agree_bitwise_and_match_the_oracle(mode, dims):
     """Every tile choice of the strided-axis kernels (narrow, 2x, 4x wide, one
     field per workgroup, autotuned) gives bit-identical results; the narrow
     run is checked against the oracle."""
-    shot, model = small_case(64, 96, 40, 6, 12, O.SRC_EXPLOSIVE, True)
+    shot, model = small_case(*dims, 6, 12, O.SRC_EXPLOSIVE, True)
     with engine_for(shot, model) as e:
         e.set_autotune(mode)
         e.begin(shot["nt"], shot["rik"], [shot_of(shot)])

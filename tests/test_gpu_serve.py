@@ -147,3 +147,73 @@ def test_cancel():
         assert s.wait(short_id, 120.0) == 0
         rc, _, rec = s.result(short_id)
         assert rc == 0 and np.array_equal(rec, plain(REQS[0]))
+
+
+def _serve_inputs(k, nt=None):
+    """Grid, model, shot and wavelet of serving-grid request k
+    (tests/fullsize_cases.py), optionally with another step count."""
+    import fullsize_cases as FC
+
+    g, m, shot = FC.serve_case(k)
+    nt = nt or shot["nt"]
+    rik = shot["rik"] if nt == shot["nt"] else qvti.ricker(nt, g["dt"], g["f0"])
+    grid = qvti.Grid(g["nx"], g["ny"], g["nz"], g["pml"], g["dx"], g["dy"], g["dz"], g["dt"], g["f0"], g["w0"],
+                     int(g["vti"]))
+    return grid, m, FC.make_shot(shot), nt, rik
+
+
+def test_every_serving_grid_matches_its_fixture_in_a_mix():
+    """One request on each serving grid (96^3 isotropic, 128^3 VTI,
+    160x160x128 isotropic, 200x200x100 VTI; 150 steps of 0.8 ms; receivers
+    off the source cell), all four in flight together: each gather within
+    1e-5 of the FP32 and FP64 oracle fixtures (tests/golden/serve_grids.npz)."""
+    import os
+
+    import fullsize_cases as FC
+
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "serve_grids.npz"))
+    with qvti.Server(1 << 40, max_active=4, steps_per_slice=25) as s:
+        ids = []
+        for k in range(len(FC.SERVE_CASES)):
+            grid, m, sh, nt, rik = _serve_inputs(k)
+            assert FC.model_checksum(m) == gold["model_sum_%d" % k]
+            ids.append(s.submit(grid, nt, rik, m, sh))
+        for k, i in enumerate(ids):
+            assert s.wait(i, 300.0) == 0, qvti.lib().qvti_last_error()
+            rc, _, rec = s.result(i)
+            assert rc == 0
+            e32 = rel_l2(rec, gold["rec_f32_%d" % k])
+            e64 = rel_l2(rec, gold["rec_f64_%d" % k])
+            print("served %s %s: vs FP32 oracle %.2e, vs FP64 oracle %.2e" % (
+                "x".join(map(str, FC.SERVE_CASES[k][0])), "VTI" if FC.SERVE_CASES[k][1] else "iso", e32, e64),
+                flush=True)
+            assert e32 < 1e-5 and e64 < 1e-5, k
+
+
+def test_long_request_is_the_same_alone_and_in_the_mix():
+    """A 1000-step request on the 96^3 serving grid is bit-identical served
+    alone and served while requests on the 128^3 and 200x200x100 grids come
+    and go."""
+    long_req = _serve_inputs(0, nt=1000)
+    with qvti.Server(1 << 40, max_active=1, steps_per_slice=50) as s:
+        grid, m, sh, nt, rik = long_req
+        i = s.submit(grid, nt, rik, m, sh)
+        assert s.wait(i, 300.0) == 0
+        alone = s.result(i)[2]
+    with qvti.Server(1 << 40, max_active=3, steps_per_slice=50) as s:
+        grid, m, sh, nt, rik = long_req
+        i = s.submit(grid, nt, rik, m, sh)
+        others = []
+        for k in (1, 3):
+            g2, m2, sh2, nt2, rik2 = _serve_inputs(k)
+            others.append(s.submit(g2, nt2, rik2, m2, sh2))
+        for o in others:
+            assert s.wait(o, 300.0) == 0
+        assert s.wait(i, 300.0) == 0
+        infos = [s.info(o)[1] for o in others] + [s.info(i)[1]]
+        mixed = s.result(i)[2]
+    assert np.isfinite(alone).all()
+    assert np.array_equal(alone, mixed)
+    # the other requests were in flight together with the long one
+    long_inf = infos[-1]
+    assert all(inf.t_admit_s < long_inf.t_done_s and long_inf.t_admit_s < inf.t_done_s for inf in infos[:-1])
