@@ -229,11 +229,12 @@ def time_steps(e, nt, rik, shots, steps, warmup):
     return time.perf_counter() - t0
 
 
-def run_batched(steps, warmup, nshots=32, nt_full=2000, widths=(8, 16, 32), workload="vti160"):
+def run_batched(steps, warmup, nshots=32, nt_full=2000, workload="vti160"):
     """32 shots of one model (default: the 160x160x128 VTI workload): the
     plain path (one shot per launch sequence) against batched launches (W
-    shots per kernel launch, W picked by timing each candidate).  shots/s is
-    for nt_full-step shots, from the timed per-step wall time of each path.
+    shots per kernel launch, W picked by the library's tuner from silent
+    timing runs of widths 1, 2, 4, ... 32).  shots/s is for nt_full-step
+    shots, from the timed per-step wall time of each path.
     At the chosen width a second region launches the step's kernels directly
     with HIP events around each launch: the sum of kernel times per step
     against the HIP-graph wall time per step is the launch overhead a
@@ -247,23 +248,19 @@ def run_batched(steps, warmup, nshots=32, nt_full=2000, widths=(8, 16, 32), work
         e.set_probe(0)
         plain_wall = time_steps(e, nt, rik, shots[:1], steps, warmup)
     plain_step = plain_wall / steps
-    trials, ksum = {}, {}
-    for w in widths:
-        if w > nshots:
-            continue
-        with engine(g, max_shots=w) as e:
-            e.set_model_dict(m)
-            e.set_probe(0)
-            wall = time_steps(e, nt, rik, shots[:w], steps, warmup)
-            trials[w] = wall / steps / w  # seconds per shot-step
-            e.enable_kernel_timing(steps)
-            e.step(steps)
-            e.sync()
-            kt = e.kernel_times()
-            ksum[w] = (sum(v["total_ms"] for v in kt.values()) / steps / 1e3, wall / steps)
-    wbest = min(trials, key=trials.get)
-    kernel_sum = (wbest,) + ksum[wbest]
-    per_shot_step = trials[wbest]
+    with engine(g, max_shots=nshots) as e:
+        e.set_model_dict(m)
+        e.set_probe(0)
+        # the library's batch-width tuner (qvti_batch_width) picks the shots per launch
+        wbest, lib_trials = e.batch_width(shots)
+        wall = time_steps(e, nt, rik, shots[:wbest], steps, warmup)
+        per_shot_step = wall / steps / wbest
+        e.enable_kernel_timing(steps)
+        e.step(steps)
+        e.sync()
+        kt = e.kernel_times()
+        kernel_sum = (wbest, sum(v["total_ms"] for v in kt.values()) / steps / 1e3, wall / steps)
+    trials = {w: ms * 1e-3 for w, ms in lib_trials.items()}
     N = g["nx"] * g["ny"] * g["nz"]
     balg = b_alg(g["nx"], g["ny"], g["nz"], g["pml"], g["vti"])
     plain_sps = 1.0 / (plain_step * nt_full)
@@ -284,6 +281,7 @@ def run_batched(steps, warmup, nshots=32, nt_full=2000, widths=(8, 16, 32), work
         "roofline_frac": {"b_alg_bytes_per_cell_step": round(balg, 1), "plain": round(gcs_plain * balg / HBM_PEAK_GBS, 4),
                           "batched": round(gcs_batch * balg / HBM_PEAK_GBS, 4)},
         "width_trials_ms_per_shot_step": {str(w): round(v * 1e3, 4) for w, v in sorted(trials.items())},
+        "width_chosen_by": "qvti_batch_width (library tuner, silent timing runs)",
     }
     w, ks, wall = kernel_sum
     out["graph_step"] = {"shots": w, "kernel_sum_ms": round(ks * 1e3, 4), "graph_wall_ms": round(wall * 1e3, 4),

@@ -102,6 +102,21 @@ int qvti_forward(qvti_ctx *c, int nt, const float *rik, const qvti_shot *shot,
 int qvti_forward_batch(qvti_ctx *c, int nt, const float *rik, const qvti_shot *shots, int nshots,
                        float *rec_vx, float *rec_vy, float *rec_vz);
 
+/* Shots per launch for nshots shots on this context: tuned by timing silent
+ * runs of widths 1, 2, 4, ... up to min(max_shots, nshots) on the context
+ * (which ends any run in progress), once per grid, medium, k-space mode,
+ * capacity and shot count, and kept for the process.  Results do not depend
+ * on the width.  Writes the width; the widths tried and their ms per
+ * shot-step go to trial_width / trial_ms (up to max_trials entries, either
+ * may be NULL).  Returns the number of widths tried.  No counterpart in the
+ * reference (one shot per GPU, viscoelas_3D.cpp:244-256). */
+int qvti_batch_width(qvti_ctx *c, const qvti_shot *shots, int nshots, int *width, int max_trials,
+                     int *trial_width, double *trial_ms);
+/* Any number of shots of the current model, run in groups of the tuned
+ * width; records as qvti_forward_batch ([shot][nt][R_s] back to back). */
+int qvti_forward_shots(qvti_ctx *c, int nt, const float *rik, const qvti_shot *shots, int nshots,
+                       float *rec_vx, float *rec_vy, float *rec_vz);
+
 /* Same argument list and meaning as the reference's cuda_forward_acoustic_3D
  * (Myfunctions.h:77-85) minus the MultiGPU plan: uploads the model, runs shot
  * ss[is] with the reference's x-force at (s_ix, s_iy, s_iz+10) and receivers
@@ -126,6 +141,46 @@ int qvti_set_snapshot(qvti_ctx *c, int field, int every, float *host_buf, long l
  * snapshots copy, ix slowest.  Default: the interior [pml, n - pml) on every
  * axis, as the reference's dump (cuviscoelas_3D.cu:1615-1627). */
 int qvti_set_snapshot_window(qvti_ctx *c, const int *lo, const int *hi);
+
+/* ---- propagation medium, the reference's media_type flag
+ * (viscoelas_3D.cpp:57-60): QVTI_MEDIA_VISCO the constant-Q scheme (default);
+ * DISPERSION its phase dispersion without the amplitude loss (tau terms
+ * dropped); LOSSLESS gamma = 0; QCOMP Q-compensating: the amplitude loss
+ * turned into a gain of the same size, the dispersion kept, for back-
+ * propagating recorded data (pair it with the wavenumber filter of
+ * qvti_set_kspace, which bounds the gain).  Applies from the next
+ * qvti_set_model. */
+enum { QVTI_MEDIA_VISCO = 0, QVTI_MEDIA_DISPERSION = 1, QVTI_MEDIA_LOSSLESS = 2, QVTI_MEDIA_QCOMP = 3 };
+int qvti_set_media(qvti_ctx *c, int media);
+
+/* ---- reverse-time migration of one shot.  The reference declares the flags
+ * and never builds the mode (RTM_type, imaging_condition, reconstruction,
+ * viscoelas_3D.cpp:42-65; checkpoints every check_steps = 500 steps,
+ * viscoelas_3D.cpp:223-229).  Imaging conditions use the reference's codes. */
+enum { QVTI_IC_EXA = 1, QVTI_IC_SRCNORM = 3, QVTI_IC_XCORR = 4 };
+enum { QVTI_FIELD_PRESSURE = 9 }; /* (pxx + pyy + pzz) / 3 */
+typedef struct {
+    int condition;        /* QVTI_IC_*: excitation amplitude R(t_ex)/S(t_ex);
+                           * source-normalised sum S R / (sum S^2 + eps max sum S^2);
+                           * cross-correlation sum S R */
+    int field;            /* imaged wavefield: QVTI_VX..QVTI_PYZ or QVTI_FIELD_PRESSURE */
+    int checkpoint_every; /* source-wavefield checkpoint interval C in steps: the
+                           * source run is recomputed in segments of C steps;
+                           * device memory 2 C nx ny nz floats (not for EXA) */
+    float eps;            /* stabilisation: source-normalised adds eps max sum S^2 to
+                           * the denominator; EXA images only where
+                           * |S(t_ex)| > eps max |S(t_ex)| */
+} qvti_rtm_opts;
+/* src and rcv: two contexts of one grid (C-PML, dt, device), each with a
+ * model set: src propagates the shot's source (rik [nt]); rcv back-propagates
+ * the records rec_* [nt][R] of the shot's R receivers (any may be NULL),
+ * injected time-reversed as forces at the receiver cells (its medium may be
+ * Q-compensating).  Image [nx][ny][nz]; illum (may be NULL): sum S^2, or for
+ * EXA the excitation amplitude S(t_ex).  Both contexts' runs end (begin them
+ * again before stepping).  QVTI_E_UNSTABLE if either wavefield blew up. */
+int qvti_rtm(qvti_ctx *src, qvti_ctx *rcv, int nt, const float *rik, const qvti_shot *shot,
+             const float *rec_vx, const float *rec_vy, const float *rec_vz, const qvti_rtm_opts *opts,
+             float *image, float *illum);
 
 /* Release everything.  Replaces cuda_Device_free (Myfunctions.h:98-102). */
 int qvti_destroy(qvti_ctx *c);

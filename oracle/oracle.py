@@ -31,7 +31,14 @@ class QoConfig(ctypes.Structure):
         ("nthreads", ctypes.c_int),
         ("ks_vref", ctypes.c_float), ("ks_fcut", ctypes.c_float), ("ks_vmax", ctypes.c_float),
         ("ks_taper", ctypes.c_float), ("slip", ctypes.c_int),
+        ("ninj", ctypes.c_int), ("inj_x", ctypes.POINTER(ctypes.c_int)), ("inj_y", ctypes.POINTER(ctypes.c_int)),
+        ("inj_z", ctypes.POINTER(ctypes.c_int)), ("inj_vx", ctypes.c_void_p), ("inj_vy", ctypes.c_void_p),
+        ("inj_vz", ctypes.c_void_p), ("hist_field", ctypes.c_int), ("hist", ctypes.c_void_p),
+        ("media", ctypes.c_int),
     ]
+
+
+PRESSURE = 9  # history field (pxx + pyy + pzz) / 3
 
 
 # planted mistakes (slip tests only): qvti_oracle.h QO_SLIP_*
@@ -127,7 +134,8 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
-def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_field=0, kspace=None, slip=0):
+def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_field=0, kspace=None, slip=0,
+        inject=None, hist_field=None, media=0):
     """Run the oracle on one shot.
 
     shot: dict(nx, ny, nz, pml, nt, dx, dy, dz, dt, f0, w0, src_kind, sx, sy, sz,
@@ -136,6 +144,11 @@ def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_fie
     kspace: optional dict(vref, fcut, vmax, taper) of the opt-in k-space
     correction / wavenumber filter (absent: the reference scheme).
     slip: SLIP_* bits of a planted mistake (slip tests only; 0: none).
+    inject: optional dict(x, y, z (int arrays [R]), vx, vy, vz (float32
+    [nt][R] or None)) of extra point sources added to v after the shot's
+    source each step; hist_field: a field id or PRESSURE whose whole grid is
+    returned at every step as out["hist"] [nt][nx][ny][nz]; media: 0
+    constant-Q, 1 dispersion only, 2 lossless, 3 Q-compensating.
     Returns dict with records [3][nt][nrec] and optionally state [9][nx][ny][nz]
     and snapshots.
     """
@@ -153,6 +166,26 @@ def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_fie
                    rz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                    vti, snap_every, snap_field, nthreads)
     cfg.slip = int(slip)
+    cfg.media = int(media)
+    keep = []
+    if inject is not None:
+        ix = [np.ascontiguousarray(inject[k], np.int32) for k in ("x", "y", "z")]
+        keep += ix
+        cfg.ninj = len(ix[0])
+        cfg.inj_x, cfg.inj_y, cfg.inj_z = [a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) for a in ix]
+        for name in ("vx", "vy", "vz"):
+            tr = inject.get(name)
+            if tr is not None:
+                tr = np.ascontiguousarray(tr, np.float32)
+                assert tr.shape == (nt, cfg.ninj)
+                keep.append(tr)
+                setattr(cfg, "inj_" + name, tr.ctypes.data)
+    hist = None
+    cfg.hist_field = -1
+    if hist_field is not None:
+        hist = np.zeros((nt, nx, ny, nz), dt)
+        cfg.hist_field = int(hist_field)
+        cfg.hist = hist.ctypes.data
     if kspace:
         cfg.ks_vref = kspace.get("vref", 0.0)
         cfg.ks_fcut = kspace.get("fcut", 0.0)
@@ -179,4 +212,6 @@ def run(shot, model, prec="f32", nthreads=0, state=False, snap_every=0, snap_fie
         out["state"] = st
     if snap_every:
         out["snaps"] = snaps
+    if hist is not None:
+        out["hist"] = hist
     return out

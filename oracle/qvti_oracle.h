@@ -55,7 +55,26 @@ typedef struct {
      * tests that show a physics check fails on a plausible misreading of the
      * reference; never set otherwise. */
     int slip;
+    /* Extra point sources (the receiver-side injection of a reverse-time
+     * migration): after the shot's source, v_c[cell r] += inj_c[it][r] for
+     * each component whose trace array [nt][ninj] is non-NULL.  Like the
+     * shot's source they enter the next step's V - V_prev. */
+    int ninj;
+    const int *inj_x, *inj_y, *inj_z;
+    const float *inj_vx, *inj_vy, *inj_vz;
+    /* Wavefield history: after the step's injections, field hist_field
+     * (QO_VX..QO_PYZ, or QO_PRESSURE = (pxx + pyy + pzz) / 3) of every cell
+     * at every step into hist [nt][N] (float or double as the build);
+     * hist_field < 0: off. */
+    int hist_field;
+    void *hist;
+    /* Propagation medium: 0 constant-Q (the scheme), 1 dispersion only (tau
+     * terms dropped), 2 lossless (gamma = 0), 3 Q-compensating (tau terms
+     * negated). */
+    int media;
 } qo_config;
+
+enum { QO_PRESSURE = 9 };
 
 /* Planted mistakes for the slip tests. */
 enum {

@@ -503,6 +503,11 @@ int FN(qo_run)(const qo_config *c, const float *rik,
     for (int r = 0; r < c->nrec; r++)
         if (c->rx[r] < 0 || c->rx[r] >= nx || c->ry[r] < 0 || c->ry[r] >= ny || c->rz[r] < 0 || c->rz[r] >= nz)
             return -1;
+    for (int r = 0; r < c->ninj; r++)
+        if (c->inj_x[r] < 0 || c->inj_x[r] >= nx || c->inj_y[r] < 0 || c->inj_y[r] >= ny || c->inj_z[r] < 0 ||
+            c->inj_z[r] >= nz)
+            return -1;
+    if (c->hist_field > QO_PRESSURE) return -1;
 #ifdef _OPENMP
     if (c->nthreads > 0) omp_set_num_threads(c->nthreads);
 #endif
@@ -560,6 +565,8 @@ int FN(qo_run)(const qo_config *c, const float *rik,
     /* planted mistakes (slip tests only): applied to every coefficient set */
 #define QSLIP(q)                                                     \
     do {                                                             \
+        if (c->media == 1) { (q)[3] = 0.0; (q)[4] = 0.0; }            \
+        if (c->media == 3) { (q)[3] = -(q)[3]; (q)[4] = -(q)[4]; }    \
         if (c->slip & QO_SLIP_ETA_SWAP) { double t_ = (q)[0]; (q)[0] = (q)[2]; (q)[2] = t_; } \
         if (c->slip & QO_SLIP_NO_TAU) { (q)[3] = 0.0; (q)[4] = 0.0; } \
         if (c->slip & QO_SLIP_ETA_SIGN) { (q)[0] = -(q)[0]; (q)[2] = -(q)[2]; } \
@@ -567,15 +574,15 @@ int FN(qo_run)(const qo_config *c, const float *rik,
 #pragma omp parallel for schedule(static)
     for (long ip = 0; ip < N; ip++) {
         double q[5];
-        qo_q_coeffs(rho_in[ip], vp[ip], gp[ip], c->w0, q);
+        qo_q_coeffs(rho_in[ip], vp[ip], c->media == 2 ? 0.0 : gp[ip], c->w0, q);
         QSLIP(q);
         for (int j = 0; j < 5; j++) cP[j][ip] = (REAL)q[j];
-        qo_q_coeffs(rho_in[ip], vs[ip], gs[ip], c->w0, q);
+        qo_q_coeffs(rho_in[ip], vs[ip], c->media == 2 ? 0.0 : gs[ip], c->w0, q);
         QSLIP(q);
         for (int j = 0; j < 5; j++) cS[j][ip] = (REAL)q[j];
         if (c->vti) {
             double vh = (eps[ip] == 0.0f) ? (double)vp[ip] : (double)vp[ip] * sqrt(1.0 + 2.0 * (double)eps[ip]);
-            qo_q_coeffs(rho_in[ip], vh, gp[ip], c->w0, q);
+            qo_q_coeffs(rho_in[ip], vh, c->media == 2 ? 0.0 : gp[ip], c->w0, q);
             QSLIP(q);
             for (int j = 0; j < 5; j++) cH[j][ip] = (REAL)q[j];
             double vd;
@@ -586,7 +593,7 @@ int FN(qo_run)(const qo_config *c, const float *rik,
                 double prod = (c33 - c44) * (c33 * (1.0 + 2.0 * (double)del[ip]) - c44);
                 vd = sqrt((sqrt(prod > 0 ? prod : 0.0) + c44) / r);
             }
-            qo_q_coeffs(rho_in[ip], vd, gp[ip], c->w0, q);
+            qo_q_coeffs(rho_in[ip], vd, c->media == 2 ? 0.0 : gp[ip], c->w0, q);
             QSLIP(q);
             for (int j = 0; j < 5; j++) cD[j][ip] = (REAL)q[j];
         }
@@ -787,11 +794,27 @@ int FN(qo_run)(const qo_config *c, const float *rik,
             case QO_SRC_FZ: vz[is] += s; break;
             default: pxx[is] += s; pyy[is] += s; pzz[is] += s; break;
             }
+            const float *inj[3] = {c->inj_vx, c->inj_vy, c->inj_vz};
+            for (int r = 0; r < c->ninj; r++) {
+                long ij = (long)c->inj_x[r] * sx + (long)c->inj_y[r] * sy + c->inj_z[r];
+                for (int f = 0; f < 3; f++)
+                    if (inj[f]) F[QO_VX + f][ij] += (REAL)inj[f][(long)it * c->ninj + r];
+            }
             for (int r = 0; r < c->nrec; r++) {
                 long ir = (long)c->rx[r] * sx + (long)c->ry[r] * sy + c->rz[r];
                 if (rec_vx) rec_vx[(long)it * c->nrec + r] = vx[ir];
                 if (rec_vy) rec_vy[(long)it * c->nrec + r] = vy[ir];
                 if (rec_vz) rec_vz[(long)it * c->nrec + r] = vz[ir];
+            }
+        }
+        /* ===== wavefield history (reverse-time migration tests) */
+        if (c->hist_field >= 0 && c->hist) {
+            REAL *dst = (REAL *)c->hist + (long)it * N;
+            if (c->hist_field == QO_PRESSURE) {
+#pragma omp parallel for schedule(static)
+                for (long ip = 0; ip < N; ip++) dst[ip] = (pxx[ip] + pyy[ip] + pzz[ip]) / (REAL)3;
+            } else {
+                memcpy(dst, F[c->hist_field], sizeof(REAL) * N);
             }
         }
         /* ===== dv <- V (get_dv_renewed, :1105-1134) */

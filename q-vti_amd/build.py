@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqvti.so")
 
-HIP_SOURCES = ["qvti_engine.hip", "qvti_serve.hip"]
+HIP_SOURCES = ["qvti_engine.hip", "qvti_serve.hip", "qvti_rtm.hip"]
 # qvti_kernels.hip is compiled once per build group of axis lengths
 # (QVTI_NGROUPS in qvti_lengths.h), the groups in parallel
 KERNEL_SOURCE = "qvti_kernels.hip"

@@ -20,7 +20,7 @@ struct RecArgs {
     const int* it;
 };
 
-__global__ __launch_bounds__(256) void k_record(RecArgs a)
+static __global__ __launch_bounds__(256) void k_record(RecArgs a)
 {
     const int shot = blockIdx.y;
     const int r0 = a.rec_off[shot], nr = a.rec_off[shot + 1] - r0;
@@ -47,7 +47,7 @@ struct SnapArgs {
     long long cap;
 };
 
-__global__ __launch_bounds__(256) void k_snapshot(SnapArgs a)
+static __global__ __launch_bounds__(256) void k_snapshot(SnapArgs a)
 {
     const int t = *a.it;
     if (a.every <= 0 || t % a.every != 0) return;
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_snapshot(SnapArgs a)
 
 // Blow-up probe: max |v| over a field (bit pattern of a non-negative float
 // orders like an int; NaN sorts above Inf).
-__global__ __launch_bounds__(256) void k_probe(const float* f, long long N, int* out)
+static __global__ __launch_bounds__(256) void k_probe(const float* f, long long N, int* out)
 {
     int m = 0;
     for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < N; q += (long long)gridDim.x * 256) {
@@ -85,41 +85,51 @@ __global__ __launch_bounds__(256) void k_probe(const float* f, long long N, int*
 // set: 0 P (vp, gp), 1 S (vs, gs), 2 H (vp*sqrt(1+2eps), gp), 3 D (v_delta, gp).
 // The tau terms act on dv = v - v_prev, so they carry the 1/dt of the
 // reference's (V - V_prev)/dt.
-__device__ __forceinline__ void q_coeffs(double rho, double v, double g, double w0, double dt, double* o)
+// media (qvti_set_media, the reference's media_type, viscoelas_3D.cpp:57-60):
+// 0 viscoelastic (as above); 1 dispersion only (tau terms dropped); 2
+// lossless (gamma = 0); 3 Q-compensating (tau terms negated: the amplitude
+// loss becomes a gain of the same size, the dispersion is kept).
+__device__ __forceinline__ void q_coeffs(double rho, double v, double g, double w0, double dt, int media, double* o)
 {
     const double pi = 3.1415926;
+    if (media == 2) g = 0.0;
     const double c = v * cos(g * pi / 2.0);
     o[0] = -rho * c * g * w0;
     o[1] = rho * (c * c);
     o[2] = rho * pow(c, 3.0) * g / w0;
     o[3] = rho * c * g * pi / dt;
     o[4] = rho * (c * c) * (g * g) * pi / w0 / dt;
+    if (media == 1) o[3] = o[4] = 0.0;
+    if (media == 3) {
+        o[3] = -o[3];
+        o[4] = -o[4];
+    }
 }
 
 // (blocked field groups and coefficient group sizes: qvti_layout.h)
 
 struct CoefArgs {
     long long N;
-    int vti;
+    int vti, media;
     double w0, dt;
     const float *vp, *gp, *vs, *gs, *rho, *eps, *del;
     float* g1;  // blocked [N/64][15][64]: P, S (S-bar filled by k_averages)
     float* g2;  // blocked [N/64][10][64]: H, D (VTI only)
 };
 
-__global__ __launch_bounds__(256) void k_coefficients(CoefArgs a)
+static __global__ __launch_bounds__(256) void k_coefficients(CoefArgs a)
 {
     for (long long ip = (long long)blockIdx.x * 256 + threadIdx.x; ip < a.N; ip += (long long)gridDim.x * 256) {
         double o[5];
         const double r = a.rho[ip];
-        q_coeffs(r, a.vp[ip], a.gp[ip], a.w0, a.dt, o);
+        q_coeffs(r, a.vp[ip], a.gp[ip], a.w0, a.dt, a.media, o);
         for (int j = 0; j < 5; ++j) a.g1[blk_index(ip, kG1, j)] = (float)o[j];
-        q_coeffs(r, a.vs[ip], a.gs[ip], a.w0, a.dt, o);
+        q_coeffs(r, a.vs[ip], a.gs[ip], a.w0, a.dt, a.media, o);
         for (int j = 0; j < 5; ++j) a.g1[blk_index(ip, kG1, 5 + j)] = (float)o[j];
         if (a.vti) {
             const double e = a.eps[ip], dl = a.del[ip], vp = a.vp[ip], vs = a.vs[ip];
             const double vh = e == 0.0 ? vp : vp * sqrt(1.0 + 2.0 * e);
-            q_coeffs(r, vh, a.gp[ip], a.w0, a.dt, o);
+            q_coeffs(r, vh, a.gp[ip], a.w0, a.dt, a.media, o);
             for (int j = 0; j < 5; ++j) a.g2[blk_index(ip, kG2, j)] = (float)o[j];
             double vd = vp;
             if (dl != 0.0) {
@@ -127,7 +137,7 @@ __global__ __launch_bounds__(256) void k_coefficients(CoefArgs a)
                 const double prod = (c33 - c44) * (c33 * (1.0 + 2.0 * dl) - c44);
                 vd = sqrt((sqrt(prod > 0.0 ? prod : 0.0) + c44) / r);
             }
-            q_coeffs(r, vd, a.gp[ip], a.w0, a.dt, o);
+            q_coeffs(r, vd, a.gp[ip], a.w0, a.dt, a.media, o);
             for (int j = 0; j < 5; ++j) a.g2[blk_index(ip, kG2, 5 + j)] = (float)o[j];
         }
     }
@@ -144,7 +154,7 @@ struct AvgArgs {
     float* buoy;     // [3][N]
 };
 
-__global__ __launch_bounds__(256) void k_averages(AvgArgs a)
+static __global__ __launch_bounds__(256) void k_averages(AvgArgs a)
 {
     for (long long ip = (long long)blockIdx.x * 256 + threadIdx.x; ip < a.N; ip += (long long)gridDim.x * 256) {
         const int ix = (int)(ip / a.sx);

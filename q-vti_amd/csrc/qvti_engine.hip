@@ -11,12 +11,15 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "qvti.h"
 #include "qvti_aux.h"
+#include "qvti_ctx.h"
 #include "qvti_internal.h"
 #include "qvti_launch.h"
 #include "qvti_pencils.h"
@@ -143,96 +146,12 @@ static void pml_profile(int n, int pml, float d, float f0, std::vector<float>& a
 
 
 // ------------------------------------------------------------ context ----
+// (struct qvti_ctx: qvti_ctx.h)
 namespace {
-enum KClass {
-    KC_TICK = 0, KC_VEL_X, KC_VEL_Y, KC_VEL_Z, KC_Y_FWD, KC_XPASS, KC_Y_INV, KC_Z_INV,
-    KC_STR_X, KC_STR_Y, KC_STR_Z, KC_RECORD, KC_SNAP, KC_PROBE, KC_COUNT
-};
+using namespace qvti;
 const char* kClassNames[KC_COUNT] = {"tick", "vel_x", "vel_y", "vel_z_fwd", "y_fwd", "xpass", "y_inv",
                                      "z_inv", "str_x", "str_y", "str_z", "record", "snapshot", "probe"};
-
-struct AxisDev {
-    int n;
-    float d;
-    float2 *tw = nullptr, *dp = nullptr, *dm = nullptr;
-    float *kf = nullptr, *a = nullptr, *b = nullptr, *ah = nullptr, *bh = nullptr;
-};
 }  // namespace
-
-struct qvti_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    qvti_grid g{};
-    long long N = 0, nrows = 0;
-    long long Nb = 0;  // blk_len(N): per-shot field length of the blocked groups
-    int S = 1;  // max shots
-    AxisDev ax[3];
-    const LenOps* ops[3] = {nullptr, nullptr, nullptr};  // kernels of the x, y, z lengths
-    // model
-    float* coef = nullptr;  // blocked coefficient groups G1 (15 fields) then G2 (10, VTI)
-    float* buoy = nullptr;  // [3][N]
-    bool has_model = false;
-    // state, field-major [field][shot][N]
-    float *v = nullptr, *sig = nullptr, *dv = nullptr, *H = nullptr, *bufA = nullptr, *bufB = nullptr;
-    float *phiv[3] = {nullptr, nullptr, nullptr}, *phis[3] = {nullptr, nullptr, nullptr};
-    long long slab[3] = {0, 0, 0};
-    int nphis[3] = {5, 5, 4};
-    // run
-    int nt = 0, nshots = 0, steps_done = 0, begun = 0;
-    float* d_rik = nullptr;
-    int* d_it = nullptr;
-    int* d_probe = nullptr;
-    int* d_src_kind = nullptr;
-    long long* d_src_ip = nullptr;
-    long long* d_rec_ip = nullptr;
-    int* d_rec_off = nullptr;
-    int total_rec = 0;
-    float* d_rec = nullptr;
-    long long rec_cap = 0;
-    int probe_every = 50;
-    float probe_limit = 1e30f;
-    // snapshot of the box [snap_lo, snap_hi) (default: the interior window)
-    int snap_field = 0, snap_every = 0;
-    int snap_lo[3] = {0, 0, 0}, snap_hi[3] = {0, 0, 0};
-    float* snap_host = nullptr;
-    long long snap_cap = 0;
-    float* d_snap = nullptr;
-    long long d_snap_alloc = 0;  // floats allocated at d_snap (only grows)
-    long long snap_lim = 0;      // floats this run may write: min(box * snapshots, snap_cap)
-    // graphs
-    bool use_graphs = true;
-    hipGraphExec_t gexec = nullptr;
-    // timing
-    bool timing = false;
-    int ev_max = 0, ev_used = 0;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> ev_class;
-    long long kc_launches[KC_COUNT] = {};
-    double kc_ms[KC_COUNT] = {};
-    // launch configuration
-    StridedArgs a_vx{}, a_vy{}, a_sx{}, a_sy{};
-    VelZArgs a_vz{};
-    ZRowsArgs a_zi{};
-    CStridedArgs a_yf{}, a_yi{};
-    XPassArgs a_xp{};
-    StressZArgs a_sz{};
-    RecArgs a_rec{};
-    dim3 g_vx, g_vy, g_vz, g_zr, g_cy, g_xp, g_sz, g_rec, g_sx, g_sy, g_yi;
-    int P[3] = {0, 0, 0};
-    // tile width per kernel class (strided-axis kernels), chosen by the autotuner
-    int pw[KC_COUNT] = {};
-    // fields per workgroup of the strided kernels: grid z = 1 (every field in
-    // one workgroup) or the field count (one field per workgroup)
-    int jz[KC_COUNT] = {};
-    bool autotune = true;
-    int tuned = 0;  // shot count the tile widths were tuned for (0: not tuned)
-    // opt-in k-space correction / wavenumber filter (off: the reference scheme)
-    bool ks = false;
-    float ks_vref = 0.f, ks_fcut = 0.f, ks_vmax = 0.f, ks_taper = 0.f;
-    float* H2 = nullptr;  // [A2 v x,y,z][A2 dv x,y,z], allocated on first use
-    long long bytes = 0;
-    std::vector<void*> allocs;
-};
 
 static int dalloc(qvti_ctx* c, void** p, size_t bytes)
 {
@@ -332,6 +251,15 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
         c->snap_hi[a] = dims[a] - g->pml;
     }
     c->nrows = (long long)g->nx * g->ny;
+    // When a z row does not end on a 128-byte line, neighbouring row tiles
+    // and neighbouring planes share lines (and 64-cell blocks): run 8 row
+    // tiles and 2 planes per XCD so those are fetched once.  Measured at
+    // 400x400x200 (800-byte rows): str_z 2472 -> 2380 us, str_x 1066 ->
+    // 1036 us; at 256^3 (aligned rows) one tile and one plane per XCD is best.
+    if ((g->nz * 4) % 128 != 0) {
+        c->run_rows = 8;
+        c->run_planes = 2;
+    }
     int rc = QVTI_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -404,8 +332,6 @@ extern "C" int qvti_destroy(qvti_ctx* c)
     return QVTI_OK;
 }
 
-// Pointers into the field-major state for shot 0.
-static float* fld(float* base, int k, const qvti_ctx* c) { return base + (long long)k * c->S * c->N; }
 
 static int build_launch_config(qvti_ctx* c)
 {
@@ -431,7 +357,7 @@ static int build_launch_config(qvti_ctx* c)
         a.axis = axis;
         a.ztiles = (nz + 2 * P - 1) / (2 * P);
         a.nplanes = axis == 0 ? ny : nx;
-        a.xcd = 1;
+        a.run = 1;
         a.dt = g.dt;
         a.shot_stride = N;
         a.slab_stride = c->slab[axis];
@@ -517,7 +443,7 @@ static int build_launch_config(qvti_ctx* c)
     yf.axis = 1;
     yf.ztiles = (nz + Py - 1) / Py;
     yf.nplanes = nx;
-    yf.xcd = 1;
+    yf.run = 1;
     yf.shot_stride = N;
     yf.tw = Y.tw;
     yf.nfields = 3;
@@ -526,7 +452,7 @@ static int build_launch_config(qvti_ctx* c)
     XPassArgs& xp = c->a_xp;
     xp.ny = ny; xp.nz = nz; xp.sx = sx; xp.sy = sy;
     xp.ztiles = (nz + Px - 1) / Px;
-    xp.xcd = 1;
+    xp.run = 1;
     xp.shot_stride = N;
     xp.tw = X.tw;
     xp.kfx = X.kf; xp.kfy = Y.kf; xp.kfz = Z.kf;
@@ -700,6 +626,7 @@ extern "C" int qvti_set_model(qvti_ctx* c, const float* velp, const float* gama_
         CoefArgs ca;
         ca.N = N;
         ca.vti = c->g.vti;
+        ca.media = c->media;
         ca.w0 = (double)c->g.w0;
         ca.dt = (double)c->g.dt;
         ca.vp = tmp; ca.gp = tmp + N; ca.vs = tmp + 2 * N; ca.gs = tmp + 3 * N; ca.rho = tmp + 4 * N;
@@ -740,10 +667,11 @@ static int record_event(qvti_ctx* c, int cls)
 #define QVTI_ZINV_SPLIT 5
 #endif
 
-// 1-D grid of the XCD-aware row kernels (tile_shot): ceil(ntiles/8)*8*S.
-static unsigned xcd_grid(int ntiles, unsigned S)
+// 1-D grid of the XCD-aware row kernels (tile_shot): ceil(ntiles/(8 run))*8 run*S.
+static unsigned xcd_grid(int ntiles, unsigned S, int run)
 {
-    return (unsigned)((ntiles + 7) / 8) * 8u * S;
+    const int g = 8 * run;
+    return (unsigned)((ntiles + g - 1) / g) * (unsigned)g * S;
 }
 
 // Launch one kernel of class cls on the context's stream.
@@ -761,7 +689,7 @@ static void launch_class(qvti_ctx* c, int cls)
         Y.deriv[MODE_VEL_ACC][pw_index(Y, c->pw[cls])](strided(c->g_vy), st, c->a_vy);
         break;
     case KC_VEL_Z:
-        Z.vel_z(dim3(xcd_grid(c->a_vz.ntiles, S)), st, c->a_vz);
+        Z.vel_z[c->a_vz.inj[0] ? 1 : 0](dim3(xcd_grid(c->a_vz.ntiles, S, c->a_vz.run)), st, c->a_vz);
         break;
     case KC_Y_FWD:
         Y.cfft[0][pw_index(Y, c->pw[cls])](strided(c->g_cy), st, c->a_yf);
@@ -784,7 +712,7 @@ static void launch_class(qvti_ctx* c, int cls)
         Y.deriv[MODE_STORE][pw_index(Y, c->pw[cls])](strided(c->g_sy), st, c->a_sy);
         break;
     case KC_STR_Z:
-        Z.stress_z[c->g.vti ? 1 : 0](dim3(xcd_grid(c->a_sz.ntiles, S)), st, c->a_sz);
+        Z.stress_z[c->g.vti ? 1 : 0](dim3(xcd_grid(c->a_sz.ntiles, S, c->a_sz.run)), st, c->a_sz);
         break;
     case KC_RECORD:
         hipLaunchKernelGGL(k_record, dim3(c->g_rec.x, S), dim3(kThreads), 0, st, c->a_rec);
@@ -816,15 +744,22 @@ static void apply_tiles(qvti_ctx* c)
         const int w = tile_w(n, c->pw[cls]).P * cols;  // z-columns per tile
         return (nz + w - 1) / w;
     };
-    // XCD-aware grid of nplanes planes of ztiles tiles (plane_tile() with xcd = 1)
-    auto grid = [&](int nplanes, int ztiles) { return dim3((unsigned)((nplanes + 7) / 8) * 8u * ztiles); };
-    c->a_vx.ztiles = zt(nx, KC_VEL_X, 2); c->g_vx = grid(ny, c->a_vx.ztiles);
-    c->a_sx.ztiles = zt(nx, KC_STR_X, 2); c->g_sx = grid(ny, c->a_sx.ztiles);
-    c->a_vy.ztiles = zt(ny, KC_VEL_Y, 2); c->g_vy = grid(nx, c->a_vy.ztiles);
-    c->a_sy.ztiles = zt(ny, KC_STR_Y, 2); c->g_sy = grid(nx, c->a_sy.ztiles);
-    c->a_yf.ztiles = zt(ny, KC_Y_FWD, 1); c->g_cy = grid(nx, c->a_yf.ztiles);
-    c->a_yi.ztiles = zt(ny, KC_Y_INV, 1); c->g_yi = grid(nx, c->a_yi.ztiles);
-    c->a_xp.ztiles = zt(nx, KC_XPASS, 1); c->g_xp = grid(ny, c->a_xp.ztiles);
+    // XCD-aware grid of nplanes planes of ztiles tiles, `run` consecutive
+    // planes per XCD (plane_tile())
+    auto grid = [&](int nplanes, int ztiles, int run) {
+        const int g = 8 * run;
+        return dim3((unsigned)((nplanes + g - 1) / g) * (unsigned)g * ztiles);
+    };
+    const int rp = c->run_planes;
+    c->a_vx.run = c->a_sx.run = c->a_vy.run = c->a_sy.run = c->a_yf.run = c->a_yi.run = c->a_xp.run = rp;
+    c->a_vx.ztiles = zt(nx, KC_VEL_X, 2); c->g_vx = grid(ny, c->a_vx.ztiles, rp);
+    c->a_sx.ztiles = zt(nx, KC_STR_X, 2); c->g_sx = grid(ny, c->a_sx.ztiles, rp);
+    c->a_vy.ztiles = zt(ny, KC_VEL_Y, 2); c->g_vy = grid(nx, c->a_vy.ztiles, rp);
+    c->a_sy.ztiles = zt(ny, KC_STR_Y, 2); c->g_sy = grid(nx, c->a_sy.ztiles, rp);
+    c->a_yf.ztiles = zt(ny, KC_Y_FWD, 1); c->g_cy = grid(nx, c->a_yf.ztiles, rp);
+    c->a_yi.ztiles = zt(ny, KC_Y_INV, 1); c->g_yi = grid(nx, c->a_yi.ztiles, rp);
+    c->a_xp.ztiles = zt(nx, KC_XPASS, 1); c->g_xp = grid(ny, c->a_xp.ztiles, rp);
+    c->a_vz.run = c->a_sz.run = c->run_rows;
 }
 
 // Fields (jobs) one launch of a strided kernel class processes.
@@ -1312,6 +1247,123 @@ extern "C" int qvti_forward(qvti_ctx* c, int nt, const float* rik, const qvti_sh
     return qvti_forward_batch(c, nt, rik, shot, 1, rec_vx, rec_vy, rec_vz);
 }
 
+// ------------------------------------------------------- batch width ----
+// Shots per launch for batched runs, tuned once per (grid, medium, k-space
+// mode, context capacity, shot count) and kept for the process: the results
+// do not depend on it (every shot's arithmetic is the same in any batch), only
+// the throughput does.
+namespace {
+struct WidthKey {
+    int nx, ny, nz, vti, ks, cap;
+    bool operator<(const WidthKey& o) const
+    {
+        return std::tie(nx, ny, nz, vti, ks, cap) < std::tie(o.nx, o.ny, o.nz, o.vti, o.ks, o.cap);
+    }
+};
+struct WidthChoice {
+    int width;
+    std::vector<std::pair<int, double>> trials;  // (width, ms per shot-step)
+};
+std::mutex g_width_mu;
+std::map<WidthKey, WidthChoice> g_width;
+}  // namespace
+
+// ms per shot-step of W shots stepped together: a silent run (zero wavelet,
+// so the state stays zero) of 2 warm-up and 6 timed steps on the context.
+static int time_width(qvti_ctx* c, const qvti_shot* shots, int nshots, int W, double* ms)
+{
+    const int warm = 2, timed = 6;
+    std::vector<float> silent(warm + timed, 0.f);
+    std::vector<qvti_shot> sh(W);
+    for (int s = 0; s < W; ++s) sh[s] = shots[s % nshots];
+    int rc = qvti_begin(c, warm + timed, silent.data(), sh.data(), W);
+    if (rc) return rc;
+    const int probe = c->probe_every;
+    c->probe_every = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        c->probe_every = probe;
+        return qvti_fail(QVTI_E_DEVICE, "hipEventCreate failed");
+    }
+    rc = qvti_step(c, warm);
+    if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = qvti_fail(QVTI_E_DEVICE, "event record failed");
+    if (!rc) rc = qvti_step(c, timed);
+    float t = 0.f;
+    if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+        rc = qvti_fail(QVTI_E_DEVICE, "width timing failed");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    c->probe_every = probe;
+    c->begun = 0;  // the timing run is not a run the caller can continue
+    *ms = (double)t / timed / W;
+    return rc;
+}
+
+extern "C" int qvti_batch_width(qvti_ctx* c, const qvti_shot* shots, int nshots, int* width, int max_trials,
+                                int* trial_width, double* trial_ms)
+{
+    if (!c || !shots || !width || nshots < 1) return qvti_fail(QVTI_E_ARG, "bad arguments");
+    if (!c->has_model) return qvti_fail(QVTI_E_STATE, "qvti_set_model must be called first");
+    const int cap = std::min(c->S, nshots);
+    const WidthKey key{c->g.nx, c->g.ny, c->g.nz, c->g.vti, c->ks ? 1 : 0, cap};
+    WidthChoice choice;
+    bool found = false;
+    {
+        std::lock_guard<std::mutex> lk(g_width_mu);
+        auto it = g_width.find(key);
+        if (it != g_width.end()) {
+            choice = it->second;
+            found = true;
+        }
+    }
+    if (!found) {
+        std::vector<int> cand;
+        for (int w = 1; w < cap; w *= 2) cand.push_back(w);
+        cand.push_back(cap);
+        double best = 1e300;
+        choice.width = 1;
+        for (int w : cand) {
+            double ms = 0.0;
+            const int rc = time_width(c, shots, nshots, w, &ms);
+            if (rc) return rc;
+            choice.trials.push_back({w, ms});
+            if (ms < best) {
+                best = ms;
+                choice.width = w;
+            }
+        }
+        std::lock_guard<std::mutex> lk(g_width_mu);
+        g_width[key] = choice;
+    }
+    *width = choice.width;
+    const int n = std::min<int>(max_trials, (int)choice.trials.size());
+    for (int k = 0; k < n; ++k) {
+        if (trial_width) trial_width[k] = choice.trials[k].first;
+        if (trial_ms) trial_ms[k] = choice.trials[k].second;
+    }
+    return (int)choice.trials.size();
+}
+
+extern "C" int qvti_forward_shots(qvti_ctx* c, int nt, const float* rik, const qvti_shot* shots, int nshots,
+                                  float* rec_vx, float* rec_vy, float* rec_vz)
+{
+    int W = 1;
+    const int rc = qvti_batch_width(c, shots, nshots, &W, 0, nullptr, nullptr);
+    if (rc < 0) return rc;
+    long long off = 0;  // floats of the records written so far, per component
+    for (int s0 = 0; s0 < nshots; s0 += W) {
+        const int k = std::min(W, nshots - s0);
+        const int r = qvti_forward_batch(c, nt, rik, shots + s0, k, rec_vx ? rec_vx + off : nullptr,
+                                         rec_vy ? rec_vy + off : nullptr, rec_vz ? rec_vz + off : nullptr);
+        if (r) return r;
+        for (int s = s0; s < s0 + k; ++s) off += (long long)nt * shots[s].nrec;
+    }
+    return QVTI_OK;
+}
+
 extern "C" int qvti_forward_acoustic_3D(qvti_ctx* c, int myid, int is, int nt, int ntx, int nty, int ntz, int ntp,
                                         int nx, int ny, int nz, int pml, float dx, float dy, float dz, float dt,
                                         float f0, float w0, float velp_max, const float* rik, const float* velp,
@@ -1368,6 +1420,15 @@ extern "C" int qvti_set_snapshot_window(qvti_ctx* c, const int* lo, const int* h
         c->snap_hi[a] = hi[a];
     }
     c->begun = 0;
+    return QVTI_OK;
+}
+
+extern "C" int qvti_set_media(qvti_ctx* c, int media)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    if (media < QVTI_MEDIA_VISCO || media > QVTI_MEDIA_QCOMP) return qvti_fail(QVTI_E_ARG, "unknown medium %d", media);
+    c->media = media;
+    c->has_model = false;  // the coefficients are built by the next qvti_set_model
     return QVTI_OK;
 }
 

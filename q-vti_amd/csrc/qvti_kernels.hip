@@ -25,10 +25,10 @@ void l_xpass(dim3 grid, hipStream_t st, const XPassArgs& a)
 {
     hipLaunchKernelGGL((k2_xpass<G, KS>), grid, dim3(G::T), 0, st, a);
 }
-template <class G>
+template <class G, bool INJ>
 void l_vel_z(dim3 grid, hipStream_t st, const VelZArgs& a)
 {
-    hipLaunchKernelGGL(k2_vel_z<G>, grid, dim3(kThreads), 0, st, a);
+    hipLaunchKernelGGL((k2_vel_z<G, INJ>), grid, dim3(kThreads), 0, st, a);
 }
 template <class G>
 void l_zrows(dim3 grid, hipStream_t st, const ZRowsArgs& a)
@@ -100,7 +100,7 @@ LenOps make_ops()
          {cfft_fn<A, B, 1, 1>(), cfft_fn<A, B, 2, 1>(), cfft_fn<A, B, 4, 1>()}},
         {{xpass_fn<A, B, 1, false>(), xpass_fn<A, B, 2, false>(), xpass_fn<A, B, 4, false>()},
          {xpass_fn<A, B, 1, true>(), xpass_fn<A, B, 2, true>(), xpass_fn<A, B, 4, true>()}},
-        &l_vel_z<G>,
+        {&l_vel_z<G, false>, &l_vel_z<G, true>},
         &l_zrows<G>,
         {&l_stress_z<G, false>, &l_stress_z<G, true>},
         &l_deriv_rows<G>,

@@ -23,7 +23,7 @@ struct LenOps {
     void (*cfft[2][3])(dim3 grid, hipStream_t st, const CStridedArgs& a);  // [0 forward, 1 inverse]
     void (*xpass[2][3])(dim3 grid, hipStream_t st, const XPassArgs& a);    // [k-space mode off, on]
     // row (z) kernels, 256 threads
-    void (*vel_z)(dim3 grid, hipStream_t st, const VelZArgs& a);
+    void (*vel_z[2])(dim3 grid, hipStream_t st, const VelZArgs& a);  // [without, with extra point sources]
     void (*zrows)(dim3 grid, hipStream_t st, const ZRowsArgs& a);
     void (*stress_z[2])(dim3 grid, hipStream_t st, const StressZArgs& a);  // [isotropic, VTI]
     void (*deriv_rows)(dim3 grid, hipStream_t st, const RowDerivArgs& a);

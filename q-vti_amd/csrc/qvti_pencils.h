@@ -51,43 +51,53 @@ constexpr int kThreads = 256;
 enum { MODE_STORE = 0, MODE_VEL_FIRST = 1, MODE_VEL_ACC = 2 };
 
 // XCD-aware placement of (row tile, shot) on a 1-D grid of
-// ceil(ntiles/8)*8*nshots workgroups.  Workgroup ids are dealt round-robin to
-// the 8 XCDs, so the shots of one row tile get ids that differ by multiples of
-// 8: they run on the same XCD (same L2), dispatched close together, and the
-// model fields they all read (coefficients, buoyancies) come from cache after
-// the first shot.  Returns tile = -1 for the padding workgroups.
+// ceil(ntiles / (8 run)) * 8 run * nshots workgroups.  Workgroup ids are dealt
+// round-robin to the 8 XCDs; id = ((c run + k) nshots + shot) 8 + x runs tile
+// (8 c + x) run + k on XCD x.  So
+//  * the shots of one row tile run on the same XCD (same L2), dispatched
+//    close together, and the model fields they all read (coefficients,
+//    buoyancies) come from cache after the first shot;
+//  * `run` consecutive row tiles run on one XCD, one after another, so the
+//    128-byte lines (and 64-cell blocks) that neighbouring tiles share when a
+//    tile does not end on a line or block boundary are fetched from HBM once.
+// Returns tile = -1 for the padding workgroups.
 struct TileShot {
     int tile, shot;
 };
-__device__ __forceinline__ TileShot tile_shot(int ntiles, int nshots)
+__device__ __forceinline__ TileShot tile_shot(int ntiles, int nshots, int run)
 {
-    const int id = blockIdx.x, grp = 8 * nshots;
-    const int g = id / grp, r = id - g * grp;
-    const int t = g * 8 + (r & 7);
-    return {t < ntiles ? t : -1, r >> 3};
+    const int id = blockIdx.x;
+    const int x = id & 7, q = id >> 3;
+    const int ck = q / nshots, shot = q - ck * nshots;
+    const int c = ck / run, k = ck - c * run;
+    const int t = (c * 8 + x) * run + k;
+    return {t < ntiles ? t : -1, shot};
 }
 
-// (plane, z-tile) of a strided-axis workgroup.  xcd = 0: blockIdx.x =
-// plane * ztiles + tile.  xcd = 1: XCD-aware placement on a 1-D grid of
-// ceil(nplanes/8)*8*ztiles workgroups - the z-tiles of one plane get ids that
-// differ by multiples of 8, so they run on the same XCD (same L2), dispatched
-// close together, and the 128-byte lines that neighbouring tiles share when
-// a tile's rows do not end on a line boundary are fetched from HBM once.
-// plane = -1 for the padding workgroups.
+// (plane, z-tile) of a strided-axis workgroup.  run = 0: blockIdx.x =
+// plane * ztiles + tile.  run >= 1: XCD-aware placement on a 1-D grid of
+// ceil(nplanes / (8 run)) * 8 run * ztiles workgroups, id = ((c run + k)
+// ztiles + zt) 8 + x running (plane (8 c + x) run + k, z-tile zt) on XCD x:
+// the z-tiles of one plane, and `run` consecutive planes, run on the same XCD
+// (same L2) one after another, so the 128-byte lines that neighbouring tiles
+// share - within a row when a tile's rows do not end on a line boundary, and
+// between the end of a row and the start of the next plane's row - are
+// fetched from HBM once.  plane = -1 for the padding workgroups.
 struct PlaneTile {
     int plane, zt;
 };
-__device__ __forceinline__ PlaneTile plane_tile(int ztiles, int nplanes, int xcd)
+__device__ __forceinline__ PlaneTile plane_tile(int ztiles, int nplanes, int run)
 {
     const int id = blockIdx.x;
-    if (!xcd) {
+    if (!run) {
         const int pl = id / ztiles;
         return {pl, id - pl * ztiles};
     }
-    const int grp = 8 * ztiles;
-    const int g = id / grp, r = id - g * grp;
-    const int pl = g * 8 + (r & 7);
-    return {pl < nplanes ? pl : -1, r >> 3};
+    const int x = id & 7, q = id >> 3;
+    const int pk = q / ztiles, zt = q - pk * ztiles;
+    const int c = pk / run, k = pk - c * run;
+    const int pl = (c * 8 + x) * run + k;
+    return {pl < nplanes ? pl : -1, zt};
 }
 
 template <class G>
@@ -116,7 +126,7 @@ struct StridedArgs {
     int nx, ny, nz, pml;
     long long sx, sy;
     int axis;  // 0: x (stride sx), 1: y (stride sy)
-    int ztiles, nplanes, xcd;  // workgroup placement: plane_tile()
+    int ztiles, nplanes, run;  // workgroup placement: plane_tile()
     int mode;
     float dt;
     long long shot_stride, slab_stride;
@@ -146,7 +156,7 @@ void k2_deriv_strided(StridedArgs a)
     // one thread of the grid advances the step counter; the kernels of the
     // step that read it run after this one
     if (a.tick != nullptr && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid == 0) *a.tick = *a.tick + 1;
-    const PlaneTile pt = plane_tile(a.ztiles, a.nplanes, a.xcd);
+    const PlaneTile pt = plane_tile(a.ztiles, a.nplanes, a.run);
     if (pt.plane < 0) return;
     const int other = pt.plane;
     const int p = tid % G::P, jj = tid / G::P;
@@ -287,7 +297,7 @@ void k2_deriv_strided(StridedArgs a)
 // Tile: G::P rows (z-pencils).
 struct VelZArgs {
     int nx, ny, nz, pml;
-    int ntiles, nshots;  // grid: tile_shot()
+    int ntiles, nshots, run;  // grid: tile_shot()
     long long nrows;
     float dt;
     long long shot_stride, slab_stride;
@@ -305,6 +315,11 @@ struct VelZArgs {
     const long long* src_ip;
     const float* rik;
     const int* it;
+    // extra point sources added to v after the previous step (the receiver
+    // side of a reverse-time migration): dense [3][shot][N], zero away from
+    // the injection cells; they enter dv like the shot's own source
+    // (k2_vel_z<G, true> only)
+    const float* inj[3];
 };
 
 // Velocity update of one component on the thread's level-1 elements of one
@@ -315,14 +330,14 @@ struct VelZArgs {
 // row bases (v, dv, acc, bu: the row; ph: the row's z-slab of 2*pml values)
 // and never alias, which lets the compiler issue the loads of later elements
 // before the stores of earlier ones.
-template <class G>
+template <class G, bool INJ>
 __device__ __forceinline__ void vz_component(float2 (&out)[G::N1], const float2 (&d)[G::N1], const bool imag,
                                              float* __restrict__ v, float* __restrict__ dv,
                                              const float* __restrict__ acc, const float* __restrict__ bu,
                                              float* __restrict__ ph, const float* __restrict__ pa,
                                              const float* __restrict__ pb, const int j, const int pml,
                                              const float dt, const bool rowdom, const bool ok1, const int src_iz,
-                                             const float sprev)
+                                             const float sprev, const float* __restrict__ inj)
 {
     constexpr int n = G::N;
     int jl = j;  // opaque per call: nothing is shared (and held) across components
@@ -350,6 +365,7 @@ __device__ __forceinline__ void vz_component(float2 (&out)[G::N1], const float2 
         const float vnew = upd ? vold + bb * (ac + dz) : vold;
         float dvv = vnew - vold;
         if (iz == src_iz) dvv += sprev;
+        if constexpr (INJ) dvv += inj[iz];
         out[j1] = make_float2(vnew, dvv);
     }
 #pragma unroll
@@ -365,13 +381,13 @@ __device__ __forceinline__ void vz_component(float2 (&out)[G::N1], const float2 
     }
 }
 
-template <class G>
+template <class G, bool INJ = false>
 __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
     constexpr int n = G::N;
-    const TileShot ts = tile_shot(a.ntiles, a.nshots);
+    const TileShot ts = tile_shot(a.ntiles, a.nshots, a.run);
     if (ts.tile < 0) return;  // padding workgroup (whole workgroup, before any barrier)
     const int tid = threadIdx.x, shot = ts.shot;
     // level 1/B mapping (column j fastest) and level 2/A mapping (row k1 fastest)
@@ -427,10 +443,11 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
         const long long rb = ubase + lrow1;  // row base of this thread's row in the shot's fields
         const int src_iz = (kind == c && sip >= r1 * n && sip < (r1 + 1) * n) ? (int)(sip - r1 * n) : -1;
         const bool h = c == 2;
-        vz_component<G>(out, d, imag, a.v[c] + rb, a.dv[c] + rb, a.acc[c] + rb,
+        vz_component<G, INJ>(out, d, imag, a.v[c] + rb, a.dv[c] + rb, a.acc[c] + rb,
                         a.buoy[c] + (rb - (long long)shot * a.shot_stride),
                         a.phi[c] + (ok1 ? slab_row : (long long)shot * a.slab_stride),
-                        h ? a.pah : a.pa, h ? a.pbh : a.pb, j, a.pml, a.dt, rowdom, ok1, src_iz, sprev);
+                        h ? a.pah : a.pa, h ? a.pbh : a.pb, j, a.pml, a.dt, rowdom, ok1, src_iz, sprev,
+                        INJ ? a.inj[c] + rb : nullptr);
     };
     // forward z-FFT of x (first pass of the 3-D transform) into dst
     auto fft_store = [&](float2 (&x)[G::N1], float2* dstp) {
@@ -488,7 +505,7 @@ __global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
 struct CStridedArgs {
     int nz;
     long long sx, sy;
-    int axis, ztiles, nplanes, xcd;
+    int axis, ztiles, nplanes, run;  // workgroup placement: plane_tile()
     long long shot_stride;
     const float2* tw;
     int nfields;
@@ -501,7 +518,7 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
-    const PlaneTile pt = plane_tile(a.ztiles, a.nplanes, a.xcd);
+    const PlaneTile pt = plane_tile(a.ztiles, a.nplanes, a.run);
     if (pt.plane < 0) return;
     const int other = pt.plane;
     const int p = tid % G::P, jj = tid / G::P;
@@ -569,7 +586,7 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
 struct XPassArgs {
     int ny, nz;
     long long sx, sy;
-    int ztiles, xcd;  // planes: ny
+    int ztiles, run;  // planes: ny; workgroup placement: plane_tile()
     long long shot_stride;
     const float2* tw;
     const float *kfx, *kfy, *kfz;
@@ -606,7 +623,7 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
     const int tid = threadIdx.x, shot = blockIdx.y;
-    const PlaneTile pt = plane_tile(a.ztiles, a.ny, a.xcd);
+    const PlaneTile pt = plane_tile(a.ztiles, a.ny, a.run);
     if (pt.plane < 0) return;
     const int iy = pt.plane;
     const int p = tid % G::P, jj = tid / G::P;
@@ -833,7 +850,7 @@ __global__ __launch_bounds__(kThreads) void k2_deriv_rows(RowDerivArgs a)
 // are staged in LDS, then every thread updates R*nz/256 cells.
 struct StressZArgs {
     int nx, ny, nz, pml;
-    int ntiles, nshots;  // grid: tile_shot()
+    int ntiles, nshots, run;  // grid: tile_shot()
     long long nrows;
     float dt;
     long long shot_stride, slabx_stride, slaby_stride, slabz_stride;
@@ -879,7 +896,7 @@ __global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
     static_assert(G::PS >= n + 1, "staging must fit in the pencil buffer");
     __shared__ float2 lds[8 * R * G::PS];
     __shared__ float2 tws[G::TWN];
-    const TileShot ts = tile_shot(a.ntiles, a.nshots);
+    const TileShot ts = tile_shot(a.ntiles, a.nshots, a.run);
     if (ts.tile < 0) return;  // padding workgroup (whole workgroup, before any barrier)
     const int tid = threadIdx.x, shot = ts.shot;
     const int j = tid % G::N2, q1 = tid / G::N2;

@@ -29,7 +29,7 @@ EXPORTS = [
     "qvti_server_cancel", "qvti_server_clock", "qvti_server_destroy", "qvti_set_autotune", "qvti_tile_choice",
     "qvti_pad_model", "qvti_load_model_raw", "qvti_write_raw", "qvti_set_kspace",
     "qvti_checkpoint_bytes", "qvti_checkpoint", "qvti_restore", "qvti_set_snapshot_window",
-    "qvti_fft3d_noalloc",
+    "qvti_fft3d_noalloc", "qvti_batch_width", "qvti_forward_shots", "qvti_set_media", "qvti_rtm",
 ]
 
 
@@ -91,7 +91,9 @@ def lib():
             "qvti_finish": [vp, vp, vp, vp], "qvti_get_state": [vp, i, i, vp], "qvti_steps_done": [vp],
             "qvti_enable_kernel_timing": [vp, i], "qvti_kernel_times": [vp, i, vp, i, vp, vp],
             "qvti_kernel_bytes": [vp, i, vp], "qvti_set_graphs": [vp, i], "qvti_set_probe": [vp, i, f],
-            "qvti_fft3d": [i, i, i, i, vp], "qvti_fft3d_noalloc": [i, i, i, i, vp, vp], "qvti_deriv1d": [i, i, i, f, i, i, vp, vp],
+            "qvti_fft3d": [i, i, i, i, vp], "qvti_fft3d_noalloc": [i, i, i, i, vp, vp], "qvti_batch_width": [vp, vp, i, vp, i, vp, vp],
+            "qvti_forward_shots": [vp, i, vp, vp, i, vp, vp, vp],
+            "qvti_set_media": [vp, i], "qvti_rtm": [vp, vp, i, vp, vp, vp, vp, vp, vp, vp, vp], "qvti_deriv1d": [i, i, i, f, i, i, vp, vp],
             "qvti_ricker": [i, f, f, vp], "qvti_gamma_from_q": [vp, vp, ll],
             "qvti_reference_receivers": [i, i, i, i, vp, vp],
             "qvti_make_model": [i, ctypes.c_uint64, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp],
@@ -232,6 +234,31 @@ def make_shot(kind, src, rx, ry, rz):
     return s
 
 
+MEDIA_VISCO, MEDIA_DISPERSION, MEDIA_LOSSLESS, MEDIA_QCOMP = 0, 1, 2, 3
+IC_EXA, IC_SRCNORM, IC_XCORR = 1, 3, 4
+FIELD_PRESSURE = 9
+
+
+class RtmOpts(ctypes.Structure):
+    _fields_ = [("condition", ctypes.c_int), ("field", ctypes.c_int), ("checkpoint_every", ctypes.c_int),
+                ("eps", ctypes.c_float)]
+
+
+def rtm(src, rcv, nt, rik, shot, recs, condition=IC_XCORR, field=FIELD_PRESSURE, checkpoint_every=500, eps=0.01):
+    """Reverse-time migration of one shot (qvti_rtm): src, rcv are Engines of
+    one grid with models set; recs = (vx, vy, vz) [nt][R] records of the
+    shot's receivers, any None.  Returns (image, illumination) [nx][ny][nz]."""
+    rik = f32(rik)
+    rv = [None if r is None else f32(r) for r in recs]
+    img = np.zeros(src.shape, np.float32)
+    ill = np.zeros(src.shape, np.float32)
+    o = RtmOpts(condition, field, checkpoint_every, eps)
+    arr = (Shot * 1)(shot)
+    check(lib().qvti_rtm(src._c, rcv._c, nt, _p(rik), ctypes.cast(arr, ctypes.c_void_p), *[_p(r) for r in rv],
+                         ctypes.byref(o), _p(img), _p(ill)))
+    return img, ill
+
+
 class Engine:
     """One engine context (one grid, up to max_shots shots in flight)."""
 
@@ -274,6 +301,10 @@ class Engine:
 
     def set_model_dict(self, m):
         self.set_model(m["vp"], m["gp"], m["vs"], m["gs"], m["rho"], m.get("eps"), m.get("delta"))
+
+    def set_media(self, media):
+        """Propagation medium (MEDIA_*) for the next set_model."""
+        check(lib().qvti_set_media(self._c, media))
 
     def set_graphs(self, on):
         check(lib().qvti_set_graphs(self._c, int(on)))
@@ -381,6 +412,31 @@ class Engine:
         rik = f32(rik)
         arr = (Shot * len(shots))(*shots)
         check(lib().qvti_forward_batch(self._c, nt, _p(rik), ctypes.cast(arr, ctypes.c_void_p), len(shots),
+                                       _p(recs[0]), _p(recs[1]), _p(recs[2])))
+        out, o = [], 0
+        for s in shots:
+            n = s.nrec * nt
+            out.append(recs[:, o:o + n].reshape(3, nt, s.nrec))
+            o += n
+        return out
+
+    def batch_width(self, shots):
+        """(shots per launch the library uses for these shots, {width: ms per shot-step tried})."""
+        arr = (Shot * len(shots))(*shots)
+        w = ctypes.c_int(0)
+        tw = np.zeros(32, np.int32)
+        tm = np.zeros(32, np.float64)
+        k = check(lib().qvti_batch_width(self._c, ctypes.cast(arr, ctypes.c_void_p), len(shots), ctypes.byref(w), 32,
+                                         _p(tw), _p(tm)))
+        return w.value, {int(tw[j]): float(tm[j]) for j in range(k)}
+
+    def forward_shots(self, nt, rik, shots):
+        """All shots, in groups of the library's tuned width; list of [3][nt][R_s]."""
+        R = sum(s.nrec for s in shots)
+        recs = np.zeros((3, R * nt), np.float32)
+        rik = f32(rik)
+        arr = (Shot * len(shots))(*shots)
+        check(lib().qvti_forward_shots(self._c, nt, _p(rik), ctypes.cast(arr, ctypes.c_void_p), len(shots),
                                        _p(recs[0]), _p(recs[1]), _p(recs[2])))
         out, o = [], 0
         for s in shots:

@@ -83,17 +83,20 @@ BATCH_SHAPE, BATCH_SHOTS, BATCH_NT, BATCH_SEED = (160, 160, 128), 32, 200, 11
 def batch_case():
     """(grid, model, shots) of the batched configuration: sources at seeded
     uniform (x, y) inside the C-PML frame at z = pml + 1, every shot with a
-    receiver line along x at y = ny/2, z = pml + 4 (the source cells are not
-    on it)."""
+    receiver line along x through its source's y at z = pml + 4, from 30
+    cells before to 30 cells after the source (clipped to the grid; the
+    source cells are not on it)."""
     nx, ny, nz = BATCH_SHAPE
     g = grid_dict(nx, ny, nz)
     m = qvti.make_model(2, MODEL_SEED, (nx, ny, nz), SMOOTH_PARAMS, vti=True)
     rng = np.random.default_rng(BATCH_SEED)
     xs = rng.integers(PML, nx - PML, BATCH_SHOTS)
     ys = rng.integers(PML, ny - PML, BATCH_SHOTS)
-    rx = np.arange(0, nx, 2, dtype=np.int32)
-    shots = [dict(kind=qvti.SRC_FZ, src=(int(x), int(y), PML + 1), rx=rx, ry=np.full(len(rx), ny // 2, np.int32),
-                  rz=np.full(len(rx), PML + 4, np.int32)) for x, y in zip(xs, ys)]
+    shots = []
+    for x, y in zip(xs, ys):
+        rx = np.arange(max(0, int(x) - 30), min(nx, int(x) + 31), 3, dtype=np.int32)
+        shots.append(dict(kind=qvti.SRC_FZ, src=(int(x), int(y), PML + 1), rx=rx,
+                          ry=np.full(len(rx), int(y), np.int32), rz=np.full(len(rx), PML + 4, np.int32)))
     return g, m, shots
 
 

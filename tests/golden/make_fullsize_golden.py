@@ -53,7 +53,7 @@ def make_batch():
     rik = FC.wavelet(FC.BATCH_NT)
     r32, r64 = run_both(g, m, shots[0], FC.BATCH_NT, rik)
     np.savez_compressed(os.path.join(HERE, "batch_vti160.npz"), rec_f32=r32, rec_f64=r64, rik=rik,
-                        src=np.array(shots[0]["src"]), model_sum=FC.model_checksum(m))
+                        src=np.array(shots[0]["src"]), rx=shots[0]["rx"], model_sum=FC.model_checksum(m))
 
 
 def make_serve():

@@ -162,6 +162,24 @@ def test_batched_shots_equal_single_shots_bitwise():
             assert np.array_equal(single, b)
 
 
+def test_forward_shots_groups_by_the_tuned_width_and_equals_single_shots():
+    """qvti_forward_shots runs 5 shots on a 4-shot context in groups of the
+    width qvti_batch_width tuned (1, 2 or 4, then a remainder group), kept per
+    grid for the process; every shot equals its single-shot run bitwise."""
+    shot, model = small_case(20, 24, 32, 4, 20, O.SRC_FZ, True)
+    xs = [5, 7, 9, 11, 13]
+    shots = [shot_of(shot, sx=x) for x in xs]
+    with engine_for(shot, model, max_shots=4) as e:
+        w, trials = e.batch_width(shots)
+        assert w in (1, 2, 4) and sorted(trials) == [1, 2, 4], (w, trials)
+        assert all(v > 0 for v in trials.values())
+        assert e.batch_width(shots) == (w, trials)  # kept, not retimed
+        recs = e.forward_shots(shot["nt"], shot["rik"], shots)
+    with engine_for(shot, model) as e:
+        for x, r in zip(xs, recs):
+            assert np.array_equal(e.forward(shot["nt"], shot["rik"], shot_of(shot, sx=x)), r), x
+
+
 @pytest.mark.parametrize("mode,dims", [(m, (64, 96, 40)) for m in (0, 2, 4, 16, 20, 1)] +
                          [(m, (90, 144, 54)) for m in (0, 4, 20)])
 def test_tile_widths_agree_bitwise_and_match_the_oracle(mode, dims):

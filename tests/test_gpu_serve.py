@@ -191,10 +191,10 @@ def test_every_serving_grid_matches_its_fixture_in_a_mix():
 
 
 def test_long_request_is_the_same_alone_and_in_the_mix():
-    """A 1000-step request on the 96^3 serving grid is bit-identical served
-    alone and served while requests on the 128^3 and 200x200x100 grids come
-    and go."""
-    long_req = _serve_inputs(0, nt=1000)
+    """A 3000-step request on the 96^3 serving grid (the serving stream's
+    longest duration) is bit-identical served alone and served while requests
+    on the 128^3 and 200x200x100 grids come and go."""
+    long_req = _serve_inputs(0, nt=3000)
     with qvti.Server(1 << 40, max_active=1, steps_per_slice=50) as s:
         grid, m, sh, nt, rik = long_req
         i = s.submit(grid, nt, rik, m, sh)

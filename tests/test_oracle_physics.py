@@ -54,6 +54,8 @@ def _taper(nt, dt, t_arrival, half, ramp=0.02):
 QCASE = dict(nx=96, ny=48, nz=48, pml=8, d=10.0, dt=1e-3, nt=200, vp=3000.0, vs=1730.0, rho=2000.0, Q=50.0,
              f0=30.0, src=(16, 24, 24), r=(14, 34))
 Q_SLIPS = (0, O.SLIP_ETA_SWAP, O.SLIP_ETA_SIGN, O.SLIP_NO_TAU)
+# propagation media run through the same check: (slip, media) keys
+Q_MEDIA = (1, 3)  # dispersion only, Q-compensating
 
 
 def _concurrent(fn, args, threads, workers=8):
@@ -68,14 +70,15 @@ def _concurrent(fn, args, threads, workers=8):
 
 @functools.lru_cache(maxsize=None)
 def _q_runs():
-    return dict(zip(Q_SLIPS, _concurrent(_q_one, [(s,) for s in Q_SLIPS], 2)))
+    keys = [(s, 0) for s in Q_SLIPS] + [(0, m) for m in Q_MEDIA]
+    return dict(zip(keys, _concurrent(_q_one, keys, 2)))
 
 
-def _q_run(slip):
-    return _q_runs()[slip]
+def _q_run(slip, media=0):
+    return _q_runs()[(slip, media)]
 
 
-def _q_one(slip, nthreads):
+def _q_one(slip, media, nthreads):
     c = QCASE
     shape = (c["nx"], c["ny"], c["nz"])
     g = np.arctan(1 / c["Q"]) / np.pi
@@ -87,17 +90,21 @@ def _q_one(slip, nthreads):
                 dt=c["dt"], f0=c["f0"], w0=float(np.float32(2 * PI_T * c["f0"])), src_kind=O.SRC_EXPLOSIVE,
                 sx=sx, sy=sy, sz=sz, rx=np.array([sx + c["r"][0], sx + c["r"][1]]), ry=np.full(2, sy),
                 rz=np.full(2, sz), rik=_ricker(c["nt"], c["dt"], c["f0"]))
-    return O.run(shot, m, "f32", nthreads=nthreads, slip=slip)["records"][0].astype(np.float64), shot["w0"], g
+    rec = O.run(shot, m, "f32", nthreads=nthreads, slip=slip, media=media)["records"][0]
+    return rec.astype(np.float64), shot["w0"], g
 
 
-def _q_wavenumber(omega, rho, vp, g, w0, dt):
+def _q_wavenumber(omega, rho, vp, g, w0, dt, tau_scale=1.0):
     """Complex wavenumber of the scheme's P wave at real frequencies: the root
     near W/c of eta3 k^3 + (eta2 - i W' tau2) k^2 + (eta1 - i W' tau1) k -
     rho W^2 = 0, W = (2/dt) sin(w dt/2), W' = W e^{i w dt/2}, with the
-    coefficients of the reference's constant-Q model (c = v cos(pi g / 2))."""
+    coefficients of the reference's constant-Q model (c = v cos(pi g / 2)),
+    the loss terms scaled by tau_scale (0: dispersion only, -1: the
+    Q-compensating medium)."""
     c = vp * np.cos(g * np.pi / 2)
     e1, e2, e3 = -rho * c * g * w0, rho * c * c, rho * c ** 3 * g / w0
-    t1, t2 = rho * c * g * np.pi, rho * c * c * g * g * np.pi / w0
+    t1 = tau_scale * rho * c * g * np.pi
+    t2 = tau_scale * rho * c * c * g * g * np.pi / w0
     out = []
     for w in omega:
         W = 2 / dt * np.sin(w * dt / 2)
@@ -107,11 +114,11 @@ def _q_wavenumber(omega, rho, vp, g, w0, dt):
     return np.array(out)
 
 
-def _q_misfit(slip):
+def _q_misfit(slip, media=0, tau_scale=1.0):
     """Relative misfit of the measured receiver-to-receiver transfer function
     against the plane-wave Green's function over 15-55 Hz."""
     c = QCASE
-    rec, w0, g = _q_run(slip)
+    rec, w0, g = _q_run(slip, media)
     if not np.all(np.isfinite(rec)):
         return np.inf, np.inf, None
     # vx sits half a cell behind its cell along x (D- derivative of pxx)
@@ -124,7 +131,7 @@ def _q_misfit(slip):
     f = np.fft.rfftfreq(nf, c["dt"])
     band = (f >= 15) & (f <= 55)
     H = np.conj(spec[1][band] / spec[0][band])  # e^{-i w t} convention
-    k = _q_wavenumber(2 * np.pi * f[band], c["rho"], c["vp"], g, w0, c["dt"])
+    k = _q_wavenumber(2 * np.pi * f[band], c["rho"], c["vp"], g, w0, c["dt"], tau_scale)
     Hp = np.exp(1j * k * (r2 - r1)) * (1j * k * r2 - 1) * r1 ** 2 / ((1j * k * r1 - 1) * r2 ** 2)
     err = np.abs(H - Hp) / np.abs(Hp)
     return float(err.max()), float(np.sqrt(np.mean(err ** 2))), (f[band], k)
@@ -156,6 +163,41 @@ def test_constant_q_check_fails_on_planted_slips(slip):
     mx, rms, _ = _q_misfit(slip)
     print("slip %d: misfit max %.3f rms %.3f" % (slip, mx, rms))
     assert rms > 0.05
+
+
+@pytest.mark.parametrize("media,tau_scale", [(1, 0.0), (3, -1.0)])
+def test_dispersion_only_and_q_compensating_media(media, tau_scale):
+    """The oracle's media used by reverse-time migration (the reference's
+    media_type 2 and 4, viscoelas_3D.cpp:57-60): dispersion-only keeps the
+    constant-Q phase speed with no loss, the Q-compensating medium turns the
+    loss into a gain of the same size - each matches the Green's function of
+    its own dispersion relation (tau scaled by 0 or -1) within 3 %, and
+    misses the constant-Q one."""
+    mx, rms, (f, k) = _q_misfit(0, media, tau_scale)
+    other, _, _ = _q_misfit(0, media, 1.0)
+    print("media %d: misfit max %.4f rms %.4f (against the lossy model: max %.3f); Im k at 40 Hz %.2e" % (
+        media, mx, rms, other, k[np.argmin(np.abs(f - 40))].imag))
+    assert mx < 0.03 and rms < 0.015
+    assert other > 0.1
+
+
+def test_injected_sources_enter_like_the_shots_own_source():
+    """An extra point source carrying the wavelet at the source cell, with the
+    shot's own wavelet silent, gives the same run bit for bit (both are added
+    after the stress update and enter the next step's V - V_prev); the
+    lossless medium equals gamma = 0 bit for bit."""
+    from cases import small_case
+
+    shot, model = small_case(16, 20, 24, 3, 25, O.SRC_FZ, True)
+    a = O.run(shot, model, "f64", state=True)
+    silent = dict(shot, rik=np.zeros_like(shot["rik"]))
+    tr = np.asarray(shot["rik"], np.float32)[:, None]
+    inj = dict(x=[shot["sx"]], y=[shot["sy"]], z=[shot["sz"]], vx=None, vy=None, vz=tr)
+    b = O.run(silent, model, "f64", state=True, inject=inj)
+    assert np.array_equal(a["state"], b["state"]) and np.array_equal(a["records"], b["records"])
+    c = O.run(shot, model, "f64", state=True, media=2)
+    d = O.run(shot, dict(model, gp=np.zeros_like(model["gp"]), gs=np.zeros_like(model["gs"])), "f64", state=True)
+    assert np.array_equal(c["state"], d["state"])
 
 
 # ----------------------------------------------------------- reciprocity
