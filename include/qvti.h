@@ -249,6 +249,23 @@ int qvti_tile_choice(qvti_ctx *c, int max_classes, int *pencils, int *split);
  * again, as after qvti_set_snapshot.  The first call that switches it on allocates 6 more fields per
  * shot; each step then runs three more complex 3-D inverse transforms. */
 int qvti_set_kspace(qvti_ctx *c, float vref, float fcut, float vmax, float taper_ratio);
+/* Two-level FFT split of each axis (0: the default N1 x N2 of the length;
+ * 1: an alternative for lengths 64 96 100 128 160 200 256 400).  The split
+ * changes the FP32 rounding (not the scheme), so it is never picked by a
+ * run's own timing: qvti_set_split fixes it for a context; qvti_tune_split
+ * times both variants per axis on the context (silent runs, ending any run in
+ * progress), applies the faster and stores the choice for the grid (nx, ny,
+ * nz, vti) in the process table that every later qvti_create of that grid
+ * uses.  qvti_save_wisdom / qvti_load_wisdom write / merge that table as text
+ * ("qvti-wisdom 1", then "nx ny nz vti split_x split_y split_z" lines).
+ * ms_per_step (may be NULL) receives [axis][variant] timings (6 values; the
+ * entries of axes without an alternative are left unset).  No counterpart in
+ * the reference (cuFFT plans its own factorisation, cuviscoelas_3D.cu:1703). */
+int qvti_set_split(qvti_ctx *c, int axis, int variant);
+int qvti_get_split(qvti_ctx *c, int *variants);
+int qvti_tune_split(qvti_ctx *c, int *variants, double *ms_per_step);
+int qvti_save_wisdom(const char *path);
+int qvti_load_wisdom(const char *path);
 /* Blow-up probe interval in steps (0 disables; default 50) and limit on |v|. */
 int qvti_set_probe(qvti_ctx *c, int every, float limit);
 /* Device bytes this context holds. */

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <vector>
 
 #include "qvti.h"
@@ -35,6 +36,7 @@ struct qvti_ctx {
     int S = 1;  // max shots
     AxisDev ax[3];
     const LenOps* ops[3] = {nullptr, nullptr, nullptr};  // kernels of the x, y, z lengths
+    int split[3] = {0, 0, 0};                            // their split variants (len_ops)
     // model
     float* coef = nullptr;  // blocked coefficient groups G1 (15 fields) then G2 (10, VTI)
     float* buoy = nullptr;  // [3][N]

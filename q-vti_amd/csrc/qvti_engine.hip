@@ -45,11 +45,20 @@ constexpr int kMaxN = 512;
 
 // The launch records of every supported axis length (qvti_lengths.h); each
 // is defined in its build group of qvti_kernels.hip.
-const LenOps* qvti::len_ops(int n)
+const LenOps* qvti::len_ops(int n, int variant)
 {
 #define QVTI_LEN_PTR(n_, a, b, g) &kLenOps_##n_,
     static const LenOps* const all[] = {QVTI_LENGTHS(QVTI_LEN_PTR)};
 #undef QVTI_LEN_PTR
+#define QVTI_ALT_PTR(n_, a, b, g) &kLenOps_Alt_##n_,
+    static const LenOps* const alt[] = {QVTI_ALT_LENGTHS(QVTI_ALT_PTR)};
+#undef QVTI_ALT_PTR
+    if (variant == 1) {
+        for (const LenOps* o : alt)
+            if (o->n == n) return o;
+        return nullptr;
+    }
+    if (variant != 0) return nullptr;
     for (const LenOps* o : all)
         if (o->n == n) return o;
     return nullptr;
@@ -67,9 +76,8 @@ const char* qvti::supported_lengths()
 struct TileW {
     int pw, P, T;
 };
-static TileW tile_w(int n, int pw)
+static TileW tile_w(const LenOps* o, int pw)
 {
-    const LenOps* o = len_ops(n);
     if (!o) return TileW{0, 0, 0};
     const int i = pw_index(*o, pw);
     return TileW{1 << i, o->P[i], o->T[i]};
@@ -220,6 +228,25 @@ hipStream_t qvti_ctx_stream(qvti_ctx* c) { return c ? c->stream : nullptr; }
 static int build_launch_config(qvti_ctx* c);
 static void apply_tiles(qvti_ctx* c);
 
+// ------------------------------------------------------- split wisdom ---
+// Split variant per axis chosen for a grid (nx, ny, nz, vti), kept for the
+// process and written to / read from a wisdom file.  The FP32 rounding of a
+// run depends on the splits, so a context takes them from this table - a
+// stored choice, not a timing of its own.
+namespace {
+using GridKey = std::array<int, 4>;
+std::mutex g_split_mu;
+std::map<GridKey, std::array<int, 3>> g_split;
+GridKey grid_key(const qvti_grid& g) { return GridKey{g.nx, g.ny, g.nz, g.vti ? 1 : 0}; }
+}  // namespace
+
+static std::array<int, 3> stored_split(const qvti_grid& g)
+{
+    std::lock_guard<std::mutex> lk(g_split_mu);
+    auto it = g_split.find(grid_key(g));
+    return it == g_split.end() ? std::array<int, 3>{0, 0, 0} : it->second;
+}
+
 extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, int device)
 {
     if (!out || !g) return qvti_fail(QVTI_E_ARG, "NULL argument");
@@ -245,7 +272,15 @@ extern "C" int qvti_create(qvti_ctx** out, const qvti_grid* g, int max_shots, in
     c->S = max_shots;
     c->N = (long long)g->nx * g->ny * g->nz;
     c->Nb = blk_len(c->N);
-    for (int a = 0; a < 3; ++a) c->ops[a] = len_ops(dims[a]);
+    {
+        // split variants: the stored choice for this grid (qvti_tune_split,
+        // qvti_load_wisdom), else the default
+        const std::array<int, 3> sp = stored_split(*g);
+        for (int a = 0; a < 3; ++a) {
+            c->split[a] = len_ops(dims[a], sp[a]) ? sp[a] : 0;
+            c->ops[a] = len_ops(dims[a], c->split[a]);
+        }
+    }
     for (int a = 0; a < 3; ++a) {
         c->snap_lo[a] = g->pml;
         c->snap_hi[a] = dims[a] - g->pml;
@@ -339,7 +374,7 @@ static int build_launch_config(qvti_ctx* c)
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long sx = (long long)ny * nz, sy = nz, N = c->N;
     const AxisDev &X = c->ax[0], &Y = c->ax[1], &Z = c->ax[2];
-    for (int a = 0; a < 3; ++a) c->P[a] = tile_pencils(c->ax[a].n);
+    for (int a = 0; a < 3; ++a) c->P[a] = c->ops[a]->P[0];
     const int Px = c->P[0], Py = c->P[1], Pz = c->P[2];
 
     float* acc[3] = {fld(c->bufA, 0, c), fld(c->bufA, 1, c), fld(c->bufA, 2, c)};
@@ -740,8 +775,8 @@ static void enqueue_step(qvti_ctx* c)
 static void apply_tiles(qvti_ctx* c)
 {
     const int nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
-    auto zt = [&](int n, int cls, int cols) {
-        const int w = tile_w(n, c->pw[cls]).P * cols;  // z-columns per tile
+    auto zt = [&](int axis, int cls, int cols) {
+        const int w = tile_w(c->ops[axis], c->pw[cls]).P * cols;  // z-columns per tile
         return (nz + w - 1) / w;
     };
     // XCD-aware grid of nplanes planes of ztiles tiles, `run` consecutive
@@ -752,13 +787,13 @@ static void apply_tiles(qvti_ctx* c)
     };
     const int rp = c->run_planes;
     c->a_vx.run = c->a_sx.run = c->a_vy.run = c->a_sy.run = c->a_yf.run = c->a_yi.run = c->a_xp.run = rp;
-    c->a_vx.ztiles = zt(nx, KC_VEL_X, 2); c->g_vx = grid(ny, c->a_vx.ztiles, rp);
-    c->a_sx.ztiles = zt(nx, KC_STR_X, 2); c->g_sx = grid(ny, c->a_sx.ztiles, rp);
-    c->a_vy.ztiles = zt(ny, KC_VEL_Y, 2); c->g_vy = grid(nx, c->a_vy.ztiles, rp);
-    c->a_sy.ztiles = zt(ny, KC_STR_Y, 2); c->g_sy = grid(nx, c->a_sy.ztiles, rp);
-    c->a_yf.ztiles = zt(ny, KC_Y_FWD, 1); c->g_cy = grid(nx, c->a_yf.ztiles, rp);
-    c->a_yi.ztiles = zt(ny, KC_Y_INV, 1); c->g_yi = grid(nx, c->a_yi.ztiles, rp);
-    c->a_xp.ztiles = zt(nx, KC_XPASS, 1); c->g_xp = grid(ny, c->a_xp.ztiles, rp);
+    c->a_vx.ztiles = zt(0, KC_VEL_X, 2); c->g_vx = grid(ny, c->a_vx.ztiles, rp);
+    c->a_sx.ztiles = zt(0, KC_STR_X, 2); c->g_sx = grid(ny, c->a_sx.ztiles, rp);
+    c->a_vy.ztiles = zt(1, KC_VEL_Y, 2); c->g_vy = grid(nx, c->a_vy.ztiles, rp);
+    c->a_sy.ztiles = zt(1, KC_STR_Y, 2); c->g_sy = grid(nx, c->a_sy.ztiles, rp);
+    c->a_yf.ztiles = zt(1, KC_Y_FWD, 1); c->g_cy = grid(nx, c->a_yf.ztiles, rp);
+    c->a_yi.ztiles = zt(1, KC_Y_INV, 1); c->g_yi = grid(nx, c->a_yi.ztiles, rp);
+    c->a_xp.ztiles = zt(0, KC_XPASS, 1); c->g_xp = grid(ny, c->a_xp.ztiles, rp);
     c->a_vz.run = c->a_sz.run = c->run_rows;
 }
 
@@ -792,12 +827,12 @@ static int autotune_tiles(qvti_ctx* c)
     c->a_vx.tick = nullptr;
     int rc = QVTI_OK;
     for (int cls : classes) {
-        const int n = (cls == KC_VEL_X || cls == KC_XPASS || cls == KC_STR_X) ? c->g.nx : c->g.ny;
+        const LenOps* o = (cls == KC_VEL_X || cls == KC_XPASS || cls == KC_STR_X) ? c->ops[0] : c->ops[1];
         const int nf = class_fields(c, cls);
         int best_pw = 1, best_jz = 1;
         float best = 1e30f;
         for (int pw : {1, 2, 4}) {
-            if (tile_w(n, pw).pw != pw) continue;  // width not available for n
+            if (tile_w(o, pw).pw != pw) continue;  // width not available for this length
             for (int jz : {1, nf}) {
                 c->pw[cls] = pw;
                 c->jz[cls] = jz;
@@ -1432,6 +1467,123 @@ extern "C" int qvti_set_media(qvti_ctx* c, int media)
     return QVTI_OK;
 }
 
+extern "C" int qvti_set_split(qvti_ctx* c, int axis, int variant)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    if (axis < 0 || axis > 2) return qvti_fail(QVTI_E_ARG, "axis %d", axis);
+    const int n = axis == 0 ? c->g.nx : (axis == 1 ? c->g.ny : c->g.nz);
+    const LenOps* o = len_ops(n, variant);
+    if (!o) return qvti_fail(QVTI_E_ARG, "length %d has no split variant %d", n, variant);
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->begun = 0;  // the launch sequence changes: begin again
+    c->split[axis] = variant;
+    c->ops[axis] = o;
+    int pw[KC_COUNT], jz[KC_COUNT];
+    std::copy(c->pw, c->pw + KC_COUNT, pw);
+    std::copy(c->jz, c->jz + KC_COUNT, jz);
+    const int rc = build_launch_config(c);
+    if (rc) return rc;
+    if (!c->autotune) {
+        std::copy(pw, pw + KC_COUNT, c->pw);
+        std::copy(jz, jz + KC_COUNT, c->jz);
+        apply_tiles(c);
+    }
+    c->tuned = 0;
+    return QVTI_OK;
+}
+
+extern "C" int qvti_get_split(qvti_ctx* c, int* variants)
+{
+    if (!c || !variants) return qvti_fail(QVTI_E_ARG, "NULL argument");
+    for (int a = 0; a < 3; ++a) variants[a] = c->split[a];
+    return QVTI_OK;
+}
+
+static int time_width(qvti_ctx* c, const qvti_shot* shots, int nshots, int W, double* ms);
+
+extern "C" int qvti_tune_split(qvti_ctx* c, int* variants, double* ms_per_step)
+{
+    if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
+    if (!c->has_model) return qvti_fail(QVTI_E_STATE, "qvti_set_model must be called first");
+    const int dims[3] = {c->g.nx, c->g.ny, c->g.nz};
+    qvti_shot sh{};
+    sh.kind = QVTI_SRC_FZ;
+    sh.sx = dims[0] / 2;
+    sh.sy = dims[1] / 2;
+    sh.sz = dims[2] / 2;
+    // one axis at a time: time a silent run with each variant, keep the faster
+    for (int a = 0; a < 3; ++a) {
+        if (!len_ops(dims[a], 1)) continue;
+        int best_v = c->split[a];
+        double best = 1e300;
+        for (int v = 0; v < 2; ++v) {
+            int rc = qvti_set_split(c, a, v);
+            double ms = 0.0;
+            if (!rc) rc = time_width(c, &sh, 1, 1, &ms);
+            if (rc) return rc;
+            if (ms_per_step) ms_per_step[2 * a + v] = ms;
+            if (ms < best) {
+                best = ms;
+                best_v = v;
+            }
+        }
+        const int rc = qvti_set_split(c, a, best_v);
+        if (rc) return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_split_mu);
+        g_split[grid_key(c->g)] = {c->split[0], c->split[1], c->split[2]};
+    }
+    if (variants)
+        for (int a = 0; a < 3; ++a) variants[a] = c->split[a];
+    return QVTI_OK;
+}
+
+extern "C" int qvti_save_wisdom(const char* path)
+{
+    if (!path) return qvti_fail(QVTI_E_ARG, "NULL path");
+    FILE* f = fopen(path, "w");
+    if (!f) return qvti_fail(QVTI_E_ARG, "cannot write %s", path);
+    std::lock_guard<std::mutex> lk(g_split_mu);
+    fprintf(f, "qvti-wisdom 1\n");
+    for (const auto& e : g_split)
+        fprintf(f, "%d %d %d %d %d %d %d\n", e.first[0], e.first[1], e.first[2], e.first[3], e.second[0], e.second[1],
+                e.second[2]);
+    const bool ok = fclose(f) == 0;
+    return ok ? QVTI_OK : qvti_fail(QVTI_E_ARG, "writing %s failed", path);
+}
+
+extern "C" int qvti_load_wisdom(const char* path)
+{
+    if (!path) return qvti_fail(QVTI_E_ARG, "NULL path");
+    FILE* f = fopen(path, "r");
+    if (!f) return qvti_fail(QVTI_E_ARG, "cannot read %s", path);
+    char head[32] = {};
+    int version = 0;
+    if (fscanf(f, "%31s %d", head, &version) != 2 || strcmp(head, "qvti-wisdom") != 0 || version != 1) {
+        fclose(f);
+        return qvti_fail(QVTI_E_ARG, "%s is not a qvti wisdom file", path);
+    }
+    std::map<GridKey, std::array<int, 3>> add;
+    int v[7];
+    int got;
+    while ((got = fscanf(f, "%d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6])) == 7) {
+        for (int a = 0; a < 3; ++a)
+            if (!len_ops(v[a], 0) || !len_ops(v[a], v[4 + a])) {
+                fclose(f);
+                return qvti_fail(QVTI_E_ARG, "%s: no split variant %d for length %d", path, v[4 + a], v[a]);
+            }
+        add[GridKey{v[0], v[1], v[2], v[3] ? 1 : 0}] = {v[4], v[5], v[6]};
+    }
+    const bool clean = got == EOF;
+    fclose(f);
+    if (!clean) return qvti_fail(QVTI_E_ARG, "%s: malformed line", path);
+    std::lock_guard<std::mutex> lk(g_split_mu);
+    for (const auto& e : add) g_split[e.first] = e.second;
+    return QVTI_OK;
+}
+
 extern "C" int qvti_set_graphs(qvti_ctx* c, int on)
 {
     if (!c) return qvti_fail(QVTI_E_ARG, "NULL context");
@@ -1504,11 +1656,11 @@ extern "C" int qvti_tile_choice(qvti_ctx* c, int max_classes, int* pencils, int*
     if (!c || !pencils) return qvti_fail(QVTI_E_ARG, "NULL argument");
     const int n = std::min(max_classes, (int)KC_COUNT);
     for (int k = 0; k < n; ++k) {
-        int len = 0;
-        if (k == KC_VEL_X || k == KC_XPASS || k == KC_STR_X) len = c->g.nx;
-        if (k == KC_VEL_Y || k == KC_Y_FWD || k == KC_Y_INV || k == KC_STR_Y) len = c->g.ny;
-        pencils[k] = len ? tile_w(len, c->pw[k]).P : 0;
-        if (split) split[k] = len ? c->jz[k] : 0;
+        const LenOps* o = nullptr;
+        if (k == KC_VEL_X || k == KC_XPASS || k == KC_STR_X) o = c->ops[0];
+        if (k == KC_VEL_Y || k == KC_Y_FWD || k == KC_Y_INV || k == KC_STR_Y) o = c->ops[1];
+        pencils[k] = o ? tile_w(o, c->pw[k]).P : 0;
+        if (split) split[k] = o ? c->jz[k] : 0;
     }
     return n;
 }

@@ -155,4 +155,7 @@ LenOps make_ops()
 
 QVTI_LENGTHS(QVTI_DEFINE_IN_GROUP)
 
+#define QVTI_ALT_IN_GROUP(n, a, b, g) QVTI_DEFINE_IN_GROUP_##g(Alt_##n, a, b)
+QVTI_ALT_LENGTHS(QVTI_ALT_IN_GROUP)
+
 }  // namespace qvti

@@ -29,8 +29,9 @@ struct LenOps {
     void (*deriv_rows)(dim3 grid, hipStream_t st, const RowDerivArgs& a);
 };
 
-// The record of axis length n, or null when n is not supported.
-const LenOps* len_ops(int n);
+// The record of axis length n with split variant 0 (default) or 1 (the
+// alternative of QVTI_ALT_LENGTHS), or null when there is none.
+const LenOps* len_ops(int n, int variant = 0);
 // The supported lengths, space separated.
 const char* supported_lengths();
 // Widest available tile width index <= the requested width pw (1, 2 or 4).
@@ -45,5 +46,8 @@ inline int pw_index(const LenOps& o, int pw)
 #define QVTI_DECLARE_LEN(n, a, b, g) extern const LenOps kLenOps_##n;
 QVTI_LENGTHS(QVTI_DECLARE_LEN)
 #undef QVTI_DECLARE_LEN
+#define QVTI_DECLARE_ALT(n, a, b, g) extern const LenOps kLenOps_Alt_##n;
+QVTI_ALT_LENGTHS(QVTI_DECLARE_ALT)
+#undef QVTI_DECLARE_ALT
 
 }  // namespace qvti

@@ -23,5 +23,12 @@
     X(384, 16, 24, 6) X(400, 20, 20, 7) X(432, 18, 24, 0) X(450, 18, 25, 1) X(480, 20, 24, 2)       \
     X(486, 18, 27, 3) X(500, 20, 25, 4) X(512, 16, 32, 5)
 
+// Alternative splits (variant 1) of the benchmark configurations' lengths:
+// the transposed order N2 x N1 where the default is not square, an unbalanced
+// split where it is.  Chosen per axis by qvti_set_split or the split tuner.
+#define QVTI_ALT_LENGTHS(X)                                                                           \
+    X(64, 4, 16, 0) X(96, 12, 8, 1) X(100, 5, 20, 2) X(128, 16, 8, 3) X(160, 16, 10, 4)               \
+    X(200, 20, 10, 5) X(256, 8, 32, 6) X(400, 16, 25, 7)
+
 // number of build groups (translation units of qvti_kernels.hip)
 #define QVTI_NGROUPS 8

@@ -30,6 +30,7 @@ EXPORTS = [
     "qvti_pad_model", "qvti_load_model_raw", "qvti_write_raw", "qvti_set_kspace",
     "qvti_checkpoint_bytes", "qvti_checkpoint", "qvti_restore", "qvti_set_snapshot_window",
     "qvti_fft3d_noalloc", "qvti_batch_width", "qvti_forward_shots", "qvti_set_media", "qvti_rtm",
+    "qvti_set_split", "qvti_get_split", "qvti_tune_split", "qvti_save_wisdom", "qvti_load_wisdom",
 ]
 
 
@@ -93,7 +94,9 @@ def lib():
             "qvti_kernel_bytes": [vp, i, vp], "qvti_set_graphs": [vp, i], "qvti_set_probe": [vp, i, f],
             "qvti_fft3d": [i, i, i, i, vp], "qvti_fft3d_noalloc": [i, i, i, i, vp, vp], "qvti_batch_width": [vp, vp, i, vp, i, vp, vp],
             "qvti_forward_shots": [vp, i, vp, vp, i, vp, vp, vp],
-            "qvti_set_media": [vp, i], "qvti_rtm": [vp, vp, i, vp, vp, vp, vp, vp, vp, vp, vp], "qvti_deriv1d": [i, i, i, f, i, i, vp, vp],
+            "qvti_set_media": [vp, i], "qvti_rtm": [vp, vp, i, vp, vp, vp, vp, vp, vp, vp, vp],
+            "qvti_set_split": [vp, i, i], "qvti_get_split": [vp, vp], "qvti_tune_split": [vp, vp, vp],
+            "qvti_save_wisdom": [ctypes.c_char_p], "qvti_load_wisdom": [ctypes.c_char_p], "qvti_deriv1d": [i, i, i, f, i, i, vp, vp],
             "qvti_ricker": [i, f, f, vp], "qvti_gamma_from_q": [vp, vp, ll],
             "qvti_reference_receivers": [i, i, i, i, vp, vp],
             "qvti_make_model": [i, ctypes.c_uint64, i, i, i, i, vp, vp, vp, vp, vp, vp, vp, vp],
@@ -189,6 +192,14 @@ def make_model(kind, seed, shape, params, nlay=1, vti=False):
     if vti:
         out["eps"], out["delta"] = e, dl
     return out
+
+
+def save_wisdom(path):
+    check(lib().qvti_save_wisdom(os.fsencode(path)))
+
+
+def load_wisdom(path):
+    check(lib().qvti_load_wisdom(os.fsencode(path)))
 
 
 def fft3d(x, sign=-1):
@@ -301,6 +312,21 @@ class Engine:
 
     def set_model_dict(self, m):
         self.set_model(m["vp"], m["gp"], m["vs"], m["gs"], m["rho"], m.get("eps"), m.get("delta"))
+
+    def set_split(self, axis, variant):
+        check(lib().qvti_set_split(self._c, axis, variant))
+
+    def split(self):
+        v = np.zeros(3, np.int32)
+        check(lib().qvti_get_split(self._c, _p(v)))
+        return tuple(int(x) for x in v)
+
+    def tune_split(self):
+        """(chosen variants per axis, {(axis, variant): ms per step})."""
+        v = np.zeros(3, np.int32)
+        ms = np.full(6, np.nan)
+        check(lib().qvti_tune_split(self._c, _p(v), _p(ms)))
+        return tuple(int(x) for x in v), {(a // 2, a % 2): float(ms[a]) for a in range(6) if np.isfinite(ms[a])}
 
     def set_media(self, media):
         """Propagation medium (MEDIA_*) for the next set_model."""

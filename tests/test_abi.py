@@ -163,3 +163,26 @@ def test_raw_model_and_record_files_round_trip(tmp_path):
     out = str(tmp_path / "rec.bin")
     qvti.write_raw(out, rec)
     assert np.array_equal(np.fromfile(out, np.float32).reshape(40, 9), rec)
+
+
+def test_split_wisdom_file_round_trip(tmp_path):
+    """qvti_load_wisdom merges "nx ny nz vti split_x split_y split_z" lines
+    into the process table that qvti_create consults, qvti_save_wisdom
+    writes it back; a split variant a length does not have, or a malformed
+    file, is refused (no device needed)."""
+    good = tmp_path / "w.txt"
+    good.write_text("qvti-wisdom 1\n256 256 256 1 1 0 1\n400 400 200 1 0 1 0\n")
+    qvti.load_wisdom(str(good))
+    out = tmp_path / "out.txt"
+    qvti.save_wisdom(str(out))
+    lines = out.read_text().split("\n")
+    assert lines[0] == "qvti-wisdom 1"
+    assert "256 256 256 1 1 0 1" in lines and "400 400 200 1 0 1 0" in lines
+    bad = tmp_path / "bad.txt"
+    bad.write_text("qvti-wisdom 1\n90 90 90 0 1 0 0\n")  # 90 has no alternative split
+    with pytest.raises(qvti.QvtiError) as err:
+        qvti.load_wisdom(str(bad))
+    assert err.value.code == qvti.E_ARG
+    bad.write_text("not wisdom\n")
+    with pytest.raises(qvti.QvtiError):
+        qvti.load_wisdom(str(bad))

@@ -206,3 +206,48 @@ def test_tile_widths_agree_bitwise_and_match_the_oracle(mode, dims):
     if mode & 6:
         assert all(tiles[k] > tiles0[k] for k in tiles0), (tiles, tiles0)  # wider tiles were used
     assert np.array_equal(rec, ref), (mode, tiles)
+
+
+def test_alternative_splits_match_the_oracle_and_change_only_rounding():
+    """The alternative two-level split of every axis (64 = 4 x 16, 96 = 12 x 8,
+    128 = 16 x 8) computes the same scheme: within the parity bound of the
+    oracle, and not bit-identical to the default split (rounding differs)."""
+    case = (64, 96, 128, 6, 12, O.SRC_EXPLOSIVE, True)
+    shot, model = small_case(*case)
+    ref32 = O.run(shot, model, "f32", state=True)
+    ref64 = O.run(shot, model, "f64", state=True)
+    outs = {}
+    for variant in (0, 1):
+        with engine_for(shot, model) as e:
+            for a in range(3):
+                e.set_split(a, variant)
+            assert e.split() == (variant,) * 3
+            e.begin(shot["nt"], shot["rik"], [shot_of(shot)])
+            e.step(shot["nt"])
+            outs[variant] = (e.finish()[0], e.state())
+    check_parity(case, outs[1][0], outs[1][1], ref32, ref64, shot["line_a"])
+    assert not np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_tuned_split_is_stored_for_the_grid_and_reused(tmp_path):
+    """qvti_tune_split times both variants per axis and stores the choice for
+    the grid: a new context of the grid starts with it and gives the same bits
+    as a context given the same splits explicitly; the choice survives a
+    wisdom file round trip."""
+    shot, model = small_case(64, 96, 128, 6, 10, O.SRC_FZ, False)
+    with engine_for(shot, model) as e:
+        chosen, ms = e.tune_split()
+    assert sorted(ms) == [(a, v) for a in range(3) for v in range(2)] and all(t > 0 for t in ms.values())
+    print("tuned splits %s, ms per step %s" % (chosen, ms))
+    with engine_for(shot, model) as e:
+        assert e.split() == chosen
+        a = e.forward(shot["nt"], shot["rik"], shot_of(shot))
+    with engine_for(shot, model) as e:
+        for ax in range(3):
+            e.set_split(ax, chosen[ax])
+        b = e.forward(shot["nt"], shot["rik"], shot_of(shot))
+    assert np.array_equal(a, b)
+    path = tmp_path / "wisdom.txt"
+    qvti.save_wisdom(str(path))
+    assert ("%d %d %d 0 %d %d %d" % (64, 96, 128, *chosen)) in path.read_text()
+    qvti.load_wisdom(str(path))
