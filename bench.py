@@ -9,6 +9,7 @@ bandwidth probe - and the CPU baseline.  Prints ONE JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload vti256|vti400|vti160|vti128|iso64]
                   [--batched] [--served NREQ] [--tiles MODE] [--kspace VREF] [--no-cpu-baseline]
                   [--headline-only] [--copy-probe]
+  python bench.py --workload W --summarize-counters TRACE FETCH WRITE COPY_FETCH COPY_WRITE BENCH_JSON
 
 value = total cells x K / timed wall seconds / 1e9 (Gcell-steps/s) over K
 steps of the production path (HIP graph per step), bracketed by device
@@ -129,7 +130,8 @@ def run_plain(name, steps, warmup, kspace_vref=0.0, tiles=1):
     return g, wall, wall_timed, kt, setup_s, begin_s, tiles, (blow_up[0] if blow_up else None)
 
 
-KERNEL_SOURCES = ("qvti_engine.hip", "qvti_pencils.h", "qvti_fft2l.h", "qvti_aux.h")
+KERNEL_SOURCES = ("qvti_engine.hip", "qvti_ctx.h", "qvti_kernels.hip", "qvti_launch.h", "qvti_lengths.h",
+                  "qvti_layout.h", "qvti_pencils.h", "qvti_fft2l.h", "qvti_aux.h")
 
 
 def kernel_source_hash():
@@ -157,6 +159,115 @@ def measured_traffic(name, kernel):
         return round(k["fetch_bytes"] + k["write_bytes"])
     except (OSError, KeyError, ValueError):
         return None
+
+
+# Kernels of one time step in launch order (the engine's enqueue_step): the
+# name each dispatch carries in a rocprofv3 trace, and the class it times.
+STEP_SIGNATURE = (("k2_deriv_strided", "vel_x"), ("k2_deriv_strided", "vel_y"), ("k2_vel_z", "vel_z_fwd"),
+                  ("k2_cfft_strided", "y_fwd"), ("k2_xpass", "xpass"), ("k2_cfft_strided", "y_inv"),
+                  ("k2_zrows", "z_inv"), ("k2_deriv_strided", "str_x"), ("k2_deriv_strided", "str_y"),
+                  ("k2_stress_z", "str_z"), ("k_record", "record"))
+
+
+def _rocprof_rows(d, suffix):
+    """Rows of every rocprofv3 CSV under directory d whose name ends in suffix."""
+    import csv
+    import glob
+
+    rows = []
+    for f in sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)):
+        with open(f, newline="") as fh:
+            rows += list(csv.DictReader(fh))
+    if not rows:
+        raise SystemExit("no *%s under %s" % (suffix, d))
+    return rows
+
+
+def _dispatches(rows, counter=None):
+    """[(dispatch id, kernel name, value)] in dispatch order: the duration in ns
+    for kernel-trace rows, the summed counter value for counter rows."""
+    per = {}
+    for r in rows:
+        did = int(r["Dispatch_Id"])
+        if counter is None:
+            per[did] = (r["Kernel_Name"], int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        elif r["Counter_Name"] == counter:
+            name, v = per.get(did, (r["Kernel_Name"], 0.0))
+            per[did] = (name, v + float(r["Counter_Value"]))
+    return [(d, n, v) for d, (n, v) in sorted(per.items())]
+
+
+def _step_classes(disp):
+    """Per class, the values of the dispatches that form complete time steps
+    (runs that match STEP_SIGNATURE; tuning launches outside steps are left
+    out)."""
+    out = {c: [] for _, c in STEP_SIGNATURE}
+    sig = [k for k, _ in STEP_SIGNATURE]
+    i = 0
+    while i + len(sig) <= len(disp):
+        names = [disp[i + j][1] for j in range(len(sig))]
+        if all(k in n for k, n in zip(sig, names)):
+            for j, (_, c) in enumerate(STEP_SIGNATURE):
+                out[c].append(disp[i + j][2])
+            i += len(sig)
+        else:
+            i += 1
+    return out
+
+
+def summarize_counters(workload, trace_dir, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, bench_json):
+    """Turns the rocprofv3 outputs of one workload into the committed counter
+    summary profiles/<workload>_traffic.json (read back by measured_traffic)
+    and prints a markdown table.  trace_dir: --kernel-trace --stats run of
+    bench.py; fetch_dir / write_dir: --pmc FETCH_SIZE / WRITE_SIZE runs of
+    bench.py; copy_*_dir: the same counters over `bench.py --copy-probe`;
+    bench_json: the JSON line of the traced run (algorithmic bytes)."""
+    import statistics
+
+    probe = 1024 << 20
+    copies = 4  # copy_probe: one warm-up copy and 3 timed ones
+
+    def copy_kib(d, counter):
+        v = [x for _, n, x in _dispatches(_rocprof_rows(d, "counter_collection.csv"), counter) if "opyBuffer" in n]
+        if len(v) < copies:
+            raise SystemExit("copy probe: %d copy dispatches under %s" % (len(v), d))
+        return sum(v[-copies:]) / copies
+
+    fetch_factor = probe / (1024.0 * copy_kib(copy_fetch_dir, "FETCH_SIZE"))
+    write_factor = probe / (1024.0 * copy_kib(copy_write_dir, "WRITE_SIZE"))
+    times = _step_classes(_dispatches(_rocprof_rows(trace_dir, "kernel_trace.csv")))
+    fetch = _step_classes(_dispatches(_rocprof_rows(fetch_dir, "counter_collection.csv"), "FETCH_SIZE"))
+    write = _step_classes(_dispatches(_rocprof_rows(write_dir, "counter_collection.csv"), "WRITE_SIZE"))
+    line = json.loads(open(bench_json).read().strip().splitlines()[-1])
+    kernels = {}
+    for _, c in STEP_SIGNATURE:
+        if not times[c] or not fetch[c] or not write[c] or c not in line["kernels"]:
+            continue
+        k = line["kernels"][c]
+        kernels[c] = {"fetch_bytes": 1024.0 * statistics.mean(fetch[c]) * fetch_factor,
+                      "write_bytes": 1024.0 * statistics.mean(write[c]) * write_factor,
+                      "algorithmic_bytes": k["GBps"] * k["avg_us"] * 1e3,
+                      "rocprof_avg_us": statistics.mean(times[c]) / 1e3,
+                      "bench_event_avg_us": k["avg_us"],
+                      "steps_traced": len(times[c]), "steps_counted": min(len(fetch[c]), len(write[c]))}
+    out = {"workload": workload, "kernel_sources_sha16": kernel_source_hash(), "fetch_factor": fetch_factor,
+           "write_factor": write_factor, "kernels": kernels}
+    with open(os.path.join(ROOT, "profiles", "%s_traffic.json" % workload), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print("FETCH_SIZE x %.4f, WRITE_SIZE x %.4f (copy probe, %d MiB each way)\n" % (fetch_factor, write_factor,
+                                                                                  probe >> 20))
+    print("| kernel class | rocprof avg us | bench HIP-event avg us | algorithmic MB | fetch MB | write MB "
+          "| traffic / algorithmic | algorithmic GB/s |")
+    print("|---|---|---|---|---|---|---|---|")
+    tot = 0.0
+    for c, k in kernels.items():
+        tot += k["rocprof_avg_us"]
+        print("| %s | %.1f | %.1f | %.1f | %.1f | %.1f | %.3f | %.0f |" % (
+            c, k["rocprof_avg_us"], k["bench_event_avg_us"], k["algorithmic_bytes"] / 1e6, k["fetch_bytes"] / 1e6,
+            k["write_bytes"] / 1e6, (k["fetch_bytes"] + k["write_bytes"]) / max(k["algorithmic_bytes"], 1.0),
+            k["algorithmic_bytes"] / (k["rocprof_avg_us"] * 1e3)))
+    print("\nsum of per-step kernel time (rocprof): %.0f us; bench wall time per step %.0f us (%.4f Gcell-steps/s)"
+          % (tot, line["ms_per_step"] * 1e3, line["value"]))
 
 
 def cpu_model():
@@ -436,7 +547,13 @@ def main():
                     help="tile mode of the strided kernels (qvti_set_autotune; default 1 = autotuned)")
     ap.add_argument("--kspace", type=float, default=0.0, metavar="VREF",
                     help="time the opt-in k-space corrected scheme (reference speed VREF m/s) instead")
+    ap.add_argument("--summarize-counters", nargs=6, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE",
+                                                              "BENCH_JSON"),
+                    help="write profiles/<workload>_traffic.json from rocprofv3 outputs (no GPU needed)")
     args = ap.parse_args()
+    if args.summarize_counters:
+        summarize_counters(args.workload, *args.summarize_counters)
+        return
     if args.copy_probe:
         print(json.dumps(copy_probe()), flush=True)
         return
