@@ -10,6 +10,7 @@ bandwidth probe - and the CPU baseline.  Prints ONE JSON line.
                   [--batched] [--served NREQ] [--tiles MODE] [--kspace VREF] [--no-cpu-baseline]
                   [--headline-only] [--copy-probe]
   python bench.py --workload W --summarize-counters TRACE FETCH WRITE COPY_FETCH COPY_WRITE BENCH_JSON
+  python bench.py --summarize-pmc DIR [DIR ...]
 
 value = total cells x K / timed wall seconds / 1e9 (Gcell-steps/s) over K
 steps of the production path (HIP graph per step), bracketed by device
@@ -268,6 +269,24 @@ def summarize_counters(workload, trace_dir, fetch_dir, write_dir, copy_fetch_dir
             k["algorithmic_bytes"] / (k["rocprof_avg_us"] * 1e3)))
     print("\nsum of per-step kernel time (rocprof): %.0f us; bench wall time per step %.0f us (%.4f Gcell-steps/s)"
           % (tot, line["ms_per_step"] * 1e3, line["value"]))
+
+
+def summarize_pmc(dirs):
+    """Per step kernel class, the mean per-dispatch value of every counter in
+    the rocprofv3 --pmc runs under dirs (one pass each), as a markdown table."""
+    import statistics
+
+    table = {}
+    for d in dirs:
+        rows = _rocprof_rows(d, "counter_collection.csv")
+        for counter in sorted({r["Counter_Name"] for r in rows}):
+            per = _step_classes(_dispatches(rows, counter))
+            table[counter] = {c: statistics.mean(v) for c, v in per.items() if v}
+    classes = [c for _, c in STEP_SIGNATURE if any(c in t for t in table.values())]
+    print("| counter | " + " | ".join(classes) + " |")
+    print("|---|" + "---|" * len(classes))
+    for counter, t in table.items():
+        print("| %s | " % counter + " | ".join("%.4g" % t[c] if c in t else "" for c in classes) + " |")
 
 
 def cpu_model():
@@ -550,7 +569,12 @@ def main():
     ap.add_argument("--summarize-counters", nargs=6, metavar=("TRACE", "FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE",
                                                               "BENCH_JSON"),
                     help="write profiles/<workload>_traffic.json from rocprofv3 outputs (no GPU needed)")
+    ap.add_argument("--summarize-pmc", nargs="+", metavar="DIR",
+                    help="per-class table of the counters of rocprofv3 --pmc runs of bench.py (no GPU needed)")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc)
+        return
     if args.summarize_counters:
         summarize_counters(args.workload, *args.summarize_counters)
         return

@@ -52,7 +52,8 @@ def build(verbose=False, out=None, defines=()):
     tmp = lib + ".tmp"
     _run([hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"], verbose)
     os.replace(tmp, lib)
-    build_host(lib, verbose)
+    if out is None:  # the command-line host links the in-tree library
+        build_host(lib, verbose)
     return lib
 
 

@@ -42,6 +42,14 @@ constexpr int kThreads = 256;
 #ifndef QVTI_XPASS_WAVES
 #define QVTI_XPASS_WAVES 1
 #endif
+// minimum waves per SIMD the z-row kernels are compiled for (1: no bound on
+// their registers; 4 caps them at 128 VGPRs, the occupancy their LDS allows)
+#ifndef QVTI_VELZ_WAVES
+#define QVTI_VELZ_WAVES 1
+#endif
+#ifndef QVTI_STRESSZ_WAVES
+#define QVTI_STRESSZ_WAVES 1
+#endif
 #ifndef QVTI_VZ_CHUNK
 #define QVTI_VZ_CHUNK 4
 #endif
@@ -382,7 +390,7 @@ __device__ __forceinline__ void vz_component(float2 (&out)[G::N1], const float2 
 }
 
 template <class G, bool INJ = false>
-__global__ __launch_bounds__(kThreads) void k2_vel_z(VelZArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_VELZ_WAVES))) void k2_vel_z(VelZArgs a)
 {
     __shared__ float2 lds[G::P * G::PS];
     __shared__ float2 tws[G::TWN];
@@ -571,9 +579,10 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
 // ------------------------------------------------------------------------
 // x-pass of the 3-D transform: forward x-FFT of P (already z- and
 // y-transformed), multiply by |k|^-1 / N and |k| / N (|k|^2 = kx^2+ky^2+kz^2
-// from the folded 1-D |K| tables, evaluated in FP64, cuviscoelas_3D.cu:72,
-// 188-190; the DC symbol of |k|^-1 is 0 because every use is followed by a
-// derivative that vanishes there), inverse x-FFT, write Q1, Q3.
+// from the folded 1-D |K| tables, evaluated in FP32 like the reference's
+// powf(k, -0.5) / powf(k, 0.5), cuviscoelas_3D.cu:72, 188-190; the DC
+// symbol of |k|^-1 is 0 because every use is followed by a derivative that
+// vanishes there), inverse x-FFT, write Q1, Q3.
 //
 // KS (opt-in k-space correction and wavenumber filter): every stress
 // operator A_j, A2 = 1 included, is multiplied by
@@ -637,9 +646,10 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
     const unsigned voff = (unsigned)(jj * sxi + p);
     float2* pen = lds + p * G::PS;
     load_twiddles<G>(tws, a.tw);
-    const double ky = a.kfy[iy];
-    const double kz = a.kfz[zok ? zc : 0];
-    const double kyz = ky * ky + kz * kz;
+    const float ky = a.kfy[iy];
+    const float kz = a.kfz[zok ? zc : 0];
+    const float kyz = fmaf(ky, ky, kz * kz);
+    const float invN = (float)a.invN;
     for (int comp = blockIdx.z; comp < a.ncomp; comp += gridDim.z) {
         const float2* const src = a.P[comp] + tbase;
         __syncthreads();
@@ -661,7 +671,7 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
 #pragma unroll
             for (int k2 = 0; k2 < G::N2; ++k2) {
                 const float kx = a.kfx[jl + G::N1 * k2];
-                const float F = kspace_factor(a, sqrtf((float)(kx * kx + kyz)));
+                const float F = kspace_factor(a, sqrtf(fmaf(kx, kx, kyz)));
                 b[k2] = make_float2(b[k2].x * F, b[k2].y * F);
             }
         }
@@ -675,21 +685,17 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
                 float2 c[G::N2];
 #pragma unroll
                 for (int k2 = 0; k2 < G::N2; ++k2) {
-                    const double kx = a.kfx[jl + G::N1 * k2];
-                    const double k2s = kx * kx + kyz;
-                    // 1/sqrt(k2s) from the hardware estimate and two Newton
-                    // steps (relative error far below 2^-50, so the FP32
-                    // rounding below matches that of the exact FP64 value)
-                    double y = __builtin_amdgcn_rsq(k2s);
-                    const double h = 0.5 * k2s;
-                    y = fma(y, fma(-h * y, y, 0.5), y);
-                    y = fma(y, fma(-h * y, y, 0.5), y);
+                    const float kx = a.kfx[jl + G::N1 * k2];
+                    const float k2s = fmaf(kx, kx, kyz);
+                    // 1/sqrt(k2s): the hardware estimate (1 ulp) and one
+                    // Newton step
+                    float y = __builtin_amdgcn_rsqf(k2s);
+                    y = fmaf(y, fmaf(-0.5f * k2s * y, y, 0.5f), y);
                     // (at k = 0 the estimate is inf and y is NaN: both
                     // symbols are 0 there, |k|^-1 by the convention above)
-                    double s = 0.0;
-                    if (KS && op == 2) s = a.invN;  // A2 = 1 (F already applied)
-                    else if (k2s > 0.0) s = op == 0 ? a.invN * y : a.invN * (k2s * y);
-                    const float fs = (float)s;
+                    float fs = 0.f;
+                    if (KS && op == 2) fs = invN;  // A2 = 1 (F already applied)
+                    else if (k2s > 0.f) fs = op == 0 ? invN * y : invN * (k2s * y);
                     c[k2] = make_float2(b[k2].x * fs, b[k2].y * fs);
                 }
                 inv_la<G>(c, pen, tws + G::T2OFF, jj);
@@ -888,7 +894,8 @@ struct StressZArgs {
 //   3: vx + i dvx (D-)         4: vy + i dvy (D-)         5: vz + i dvz (D+)
 //   6: A3 vx + i A3 vy (D-)    7: A3 vz (D+)
 template <class G, bool VTI>
-__global__ __launch_bounds__(kThreads) void k2_stress_z(StressZArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_STRESSZ_WAVES))) void k2_stress_z(
+    StressZArgs a)
 {
     constexpr int R = G::P / 8;
     constexpr int n = G::N;
