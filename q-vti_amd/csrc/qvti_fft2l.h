@@ -121,6 +121,10 @@ __host__ __device__ __forceinline__ float2 c_twc(float2 a, int e)
 {
     const int r = e % N;
     if (r == 0) return a;
+    // quarter turns are exact swaps and negations
+    if (4 * r == N) return c_muli<S>(a);
+    if (2 * r == N) return make_float2(-a.x, -a.y);
+    if (4 * r == 3 * N) return c_muli<-S>(a);
     const float wr = TwTab<N>::t.re[r];
     const float wi = S < 0 ? TwTab<N>::t.im[r] : -TwTab<N>::t.im[r];
     return make_float2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);

@@ -687,15 +687,15 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
                 for (int k2 = 0; k2 < G::N2; ++k2) {
                     const float kx = a.kfx[jl + G::N1 * k2];
                     const float k2s = fmaf(kx, kx, kyz);
-                    // 1/sqrt(k2s): the hardware estimate (1 ulp) and one
-                    // Newton step
-                    float y = __builtin_amdgcn_rsqf(k2s);
-                    y = fmaf(y, fmaf(-0.5f * k2s * y, y, 0.5f), y);
-                    // (at k = 0 the estimate is inf and y is NaN: both
-                    // symbols are 0 there, |k|^-1 by the convention above)
-                    float fs = 0.f;
-                    if (KS && op == 2) fs = invN;  // A2 = 1 (F already applied)
-                    else if (k2s > 0.f) fs = op == 0 ? invN * y : invN * (k2s * y);
+                    // 1/sqrt(k2s): the hardware reciprocal square root
+                    // (1 ulp, as close as the reference's powf)
+                    const float y = __builtin_amdgcn_rsqf(k2s);
+                    float fs = KS && op == 2 ? invN  // A2 = 1 (F already applied)
+                                             : (op == 0 ? invN * y : invN * (k2s * y));
+                    // k = 0 only at element 0 (k2 = 0, jl = 0), where y is
+                    // inf: both symbols are 0 there, |k|^-1 by the
+                    // convention above
+                    if (k2 == 0 && !(KS && op == 2) && !(k2s > 0.f)) fs = 0.f;
                     c[k2] = make_float2(b[k2].x * fs, b[k2].y * fs);
                 }
                 inv_la<G>(c, pen, tws + G::T2OFF, jj);
