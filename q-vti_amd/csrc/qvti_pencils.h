@@ -579,10 +579,10 @@ __global__ __launch_bounds__(G::T) void k2_cfft_strided(CStridedArgs a)
 // ------------------------------------------------------------------------
 // x-pass of the 3-D transform: forward x-FFT of P (already z- and
 // y-transformed), multiply by |k|^-1 / N and |k| / N (|k|^2 = kx^2+ky^2+kz^2
-// from the folded 1-D |K| tables, evaluated in FP32 like the reference's
-// powf(k, -0.5) / powf(k, 0.5), cuviscoelas_3D.cu:72, 188-190; the DC
-// symbol of |k|^-1 is 0 because every use is followed by a derivative that
-// vanishes there), inverse x-FFT, write Q1, Q3.
+// from the folded 1-D |K| tables, evaluated in FP64 and rounded once,
+// cuviscoelas_3D.cu:72, 188-190; the DC symbol of |k|^-1 is 0 because every
+// use is followed by a derivative that vanishes there), inverse x-FFT,
+// write Q1, Q3.
 //
 // KS (opt-in k-space correction and wavenumber filter): every stress
 // operator A_j, A2 = 1 included, is multiplied by
@@ -646,10 +646,9 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
     const unsigned voff = (unsigned)(jj * sxi + p);
     float2* pen = lds + p * G::PS;
     load_twiddles<G>(tws, a.tw);
-    const float ky = a.kfy[iy];
-    const float kz = a.kfz[zok ? zc : 0];
-    const float kyz = fmaf(ky, ky, kz * kz);
-    const float invN = (float)a.invN;
+    const double ky = a.kfy[iy];
+    const double kz = a.kfz[zok ? zc : 0];
+    const double kyz = ky * ky + kz * kz;
     for (int comp = blockIdx.z; comp < a.ncomp; comp += gridDim.z) {
         const float2* const src = a.P[comp] + tbase;
         __syncthreads();
@@ -670,8 +669,8 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
             // more FP32 rounding than folding it into each symbol)
 #pragma unroll
             for (int k2 = 0; k2 < G::N2; ++k2) {
-                const float kx = a.kfx[jl + G::N1 * k2];
-                const float F = kspace_factor(a, sqrtf(fmaf(kx, kx, kyz)));
+                const double kx = a.kfx[jl + G::N1 * k2];
+                const float F = kspace_factor(a, sqrtf((float)(kx * kx + kyz)));
                 b[k2] = make_float2(b[k2].x * F, b[k2].y * F);
             }
         }
@@ -685,17 +684,21 @@ __global__ __launch_bounds__(G::T) __attribute__((amdgpu_waves_per_eu(QVTI_XPASS
                 float2 c[G::N2];
 #pragma unroll
                 for (int k2 = 0; k2 < G::N2; ++k2) {
-                    const float kx = a.kfx[jl + G::N1 * k2];
-                    const float k2s = fmaf(kx, kx, kyz);
-                    // 1/sqrt(k2s): the hardware reciprocal square root
-                    // (1 ulp, as close as the reference's powf)
-                    const float y = __builtin_amdgcn_rsqf(k2s);
-                    float fs = KS && op == 2 ? invN  // A2 = 1 (F already applied)
-                                             : (op == 0 ? invN * y : invN * (k2s * y));
-                    // k = 0 only at element 0 (k2 = 0, jl = 0), where y is
-                    // inf: both symbols are 0 there, |k|^-1 by the
-                    // convention above
-                    if (k2 == 0 && !(KS && op == 2) && !(k2s > 0.f)) fs = 0.f;
+                    const double kx = a.kfx[jl + G::N1 * k2];
+                    const double k2s = kx * kx + kyz;
+                    // 1/sqrt(k2s): the FP32 hardware estimate (relative
+                    // error below 2^-23) and one FP64 Newton step (below
+                    // 2^-45), so the FP32 rounding below is that of the
+                    // exact value except within 2^-45 of a rounding boundary
+                    double y = __builtin_amdgcn_rsqf((float)k2s);
+                    y = fma(y, fma(-0.5 * k2s * y, y, 0.5), y);
+                    const double s = KS && op == 2 ? a.invN  // A2 = 1 (F already applied)
+                                                   : (op == 0 ? a.invN * y : a.invN * (k2s * y));
+                    float fs = (float)s;
+                    // k = 0 only at element 0 (k2 = 0, jl = 0), where the
+                    // estimate is inf and y NaN: both symbols are 0 there,
+                    // |k|^-1 by the convention above
+                    if (k2 == 0 && !(KS && op == 2) && !(k2s > 0.0)) fs = 0.f;
                     c[k2] = make_float2(b[k2].x * fs, b[k2].y * fs);
                 }
                 inv_la<G>(c, pen, tws + G::T2OFF, jj);

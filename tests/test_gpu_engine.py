@@ -117,7 +117,8 @@ def check_parity(case, rec, st, ref32, ref64, la):
         eb = rel_l2(rec[:, :, la:], ref["records"][:, :, la:])
         ea = rel_l2(rec[:, :, :la], ref["records"][:, :, :la])
         res[name] = (es, eb, ea)
-        print(case, name, "state %.2e near-line %.2e far-line %.2e (fp32 floor %.2e)" % (es, eb, ea, floor))
+        print(case, name, "state %.2e whole gather %.2e near-line %.2e far-line %.2e (fp32 floor %.2e)%s" % (
+            es, rel_l2(rec, ref["records"]), eb, ea, floor, " far line within the floor allowance" if ea >= TOL else ""))
     for name, k in (("f32", 3.0), ("f64", 2.0)):
         es, eb, ea = res[name]
         assert es < TOL and eb < TOL, (case, name, es, eb)
