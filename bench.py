@@ -390,6 +390,9 @@ def run_batched(steps, warmup, nshots=32, nt_full=2000, workload="vti160"):
         e.sync()
         kt = e.kernel_times()
         kernel_sum = (wbest, sum(v["total_ms"] for v in kt.values()) / steps / 1e3, wall / steps)
+        per_kernel = {n: {"avg_us": round(1e3 * v["total_ms"] / v["launches"], 2),
+                          "GBps": round(v["bytes_per_launch"] / (v["total_ms"] / v["launches"] * 1e-3) / 1e9, 1)}
+                      for n, v in kt.items() if v["launches"] > 0}
     trials = {w: ms * 1e-3 for w, ms in lib_trials.items()}
     N = g["nx"] * g["ny"] * g["nz"]
     balg = b_alg(g["nx"], g["ny"], g["nz"], g["pml"], g["vti"])
@@ -416,6 +419,7 @@ def run_batched(steps, warmup, nshots=32, nt_full=2000, workload="vti160"):
     w, ks, wall = kernel_sum
     out["graph_step"] = {"shots": w, "kernel_sum_ms": round(ks * 1e3, 4), "graph_wall_ms": round(wall * 1e3, 4),
                          "launch_overhead_frac": round(1.0 - ks / wall, 4)}
+    out["kernels"] = per_kernel
     return out
 
 
