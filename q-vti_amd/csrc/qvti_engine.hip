@@ -83,12 +83,6 @@ static TileW tile_w(const LenOps* o, int pw)
     return TileW{1 << i, o->P[i], o->T[i]};
 }
 
-static int tile_pencils(int n)
-{
-    const LenOps* o = len_ops(n);
-    return o ? o->P[0] : 0;
-}
-
 static bool supported_n(int n) { return len_ops(n) != nullptr; }
 
 // The reference's truncated pi (cuviscoelas_3D.cu:15).
@@ -374,7 +368,8 @@ static int build_launch_config(qvti_ctx* c)
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long long sx = (long long)ny * nz, sy = nz, N = c->N;
     const AxisDev &X = c->ax[0], &Y = c->ax[1], &Z = c->ax[2];
-    for (int a = 0; a < 3; ++a) c->P[a] = c->ops[a]->P[0];
+    for (int a = 0; a < 2; ++a) c->P[a] = c->ops[a]->P[0];
+    c->P[2] = c->ops[2]->Prow;
     const int Px = c->P[0], Py = c->P[1], Pz = c->P[2];
 
     float* acc[3] = {fld(c->bufA, 0, c), fld(c->bufA, 1, c), fld(c->bufA, 2, c)};
@@ -1756,7 +1751,7 @@ static void enqueue_fft3d(int nx, int ny, int nz, int sign, float2* d, const flo
     ZRowsArgs z{};
     z.nrows = nrows; z.shot_stride = N; z.tw = tw[2]; z.mode = 2; z.dir = sign; z.nfields = 1;
     z.cin[0] = d; z.cout[0] = d;
-    Z.zrows(dim3((unsigned)((nrows + Z.P[0] - 1) / Z.P[0]), 1), st, z);
+    Z.zrows(dim3((unsigned)((nrows + Z.Prow - 1) / Z.Prow), 1), st, z);
     for (int axis = 1; axis >= 0; --axis) {
         const LenOps& A = *len_ops(dims[axis]);
         CStridedArgs y{};
@@ -1909,7 +1904,7 @@ extern "C" int qvti_deriv1d(int nx, int ny, int nz, float dd, int axis, int sym,
             RowDerivArgs r{};
             const long long nrows = (long long)nx * ny;
             r.nrows = nrows; r.tw = dtw; r.sym = dsym; r.in = din; r.out = dout;
-            A.deriv_rows(dim3((unsigned)((nrows + 2 * P - 1) / (2 * P))), nullptr, r);
+            A.deriv_rows(dim3((unsigned)((nrows + 2 * A.Prow - 1) / (2 * A.Prow))), nullptr, r);
         }
         if (hipDeviceSynchronize() != hipSuccess || hipGetLastError() != hipSuccess)
             rc = qvti_fail(QVTI_E_DEVICE, "derivative kernel failed");

@@ -85,21 +85,25 @@ constexpr int pencils()
 
 // Built at load time (dynamic initialisation), so that every launch function
 // whose address the record holds is instantiated and emitted.
+// tile widths PW of width index 0, 1, 2
+constexpr int kW0 = 1, kW1 = 2, kW2 = 4;
+
 template <int A, int B>
 LenOps make_ops()
 {
     using G = L2<A, B, 1>;
     return LenOps{
-        A * B, A, B,
-        {pencils<A, B, 1>(), pencils<A, B, 2>(), pencils<A, B, 4>()},
-        {256, 512, 1024},
-        {{deriv_fn<A, B, 1, MODE_STORE>(), deriv_fn<A, B, 2, MODE_STORE>(), deriv_fn<A, B, 4, MODE_STORE>()},
-         {deriv_fn<A, B, 1, MODE_VEL_FIRST>(), deriv_fn<A, B, 2, MODE_VEL_FIRST>(), deriv_fn<A, B, 4, MODE_VEL_FIRST>()},
-         {deriv_fn<A, B, 1, MODE_VEL_ACC>(), deriv_fn<A, B, 2, MODE_VEL_ACC>(), deriv_fn<A, B, 4, MODE_VEL_ACC>()}},
-        {{cfft_fn<A, B, 1, -1>(), cfft_fn<A, B, 2, -1>(), cfft_fn<A, B, 4, -1>()},
-         {cfft_fn<A, B, 1, 1>(), cfft_fn<A, B, 2, 1>(), cfft_fn<A, B, 4, 1>()}},
-        {{xpass_fn<A, B, 1, false>(), xpass_fn<A, B, 2, false>(), xpass_fn<A, B, 4, false>()},
-         {xpass_fn<A, B, 1, true>(), xpass_fn<A, B, 2, true>(), xpass_fn<A, B, 4, true>()}},
+        A * B, A, B, G::P,
+        {pencils<A, B, kW0>(), pencils<A, B, kW1>(), pencils<A, B, kW2>()},
+        {L2<A, B, kW0>::T, L2<A, B, kW1>::T, L2<A, B, kW2>::T},
+        {{deriv_fn<A, B, kW0, MODE_STORE>(), deriv_fn<A, B, kW1, MODE_STORE>(), deriv_fn<A, B, kW2, MODE_STORE>()},
+         {deriv_fn<A, B, kW0, MODE_VEL_FIRST>(), deriv_fn<A, B, kW1, MODE_VEL_FIRST>(),
+          deriv_fn<A, B, kW2, MODE_VEL_FIRST>()},
+         {deriv_fn<A, B, kW0, MODE_VEL_ACC>(), deriv_fn<A, B, kW1, MODE_VEL_ACC>(), deriv_fn<A, B, kW2, MODE_VEL_ACC>()}},
+        {{cfft_fn<A, B, kW0, -1>(), cfft_fn<A, B, kW1, -1>(), cfft_fn<A, B, kW2, -1>()},
+         {cfft_fn<A, B, kW0, 1>(), cfft_fn<A, B, kW1, 1>(), cfft_fn<A, B, kW2, 1>()}},
+        {{xpass_fn<A, B, kW0, false>(), xpass_fn<A, B, kW1, false>(), xpass_fn<A, B, kW2, false>()},
+         {xpass_fn<A, B, kW0, true>(), xpass_fn<A, B, kW1, true>(), xpass_fn<A, B, kW2, true>()}},
         {&l_vel_z<G, false>, &l_vel_z<G, true>},
         &l_zrows<G>,
         {&l_stress_z<G, false>, &l_stress_z<G, true>},

@@ -16,8 +16,9 @@ namespace qvti {
 
 struct LenOps {
     int n, n1, n2;
-    int P[3];  // pencils per tile of the strided kernels at PW 1, 2, 4 (0: not available)
-    int T[3];  // threads per workgroup at PW 1, 2, 4
+    int Prow;  // rows per workgroup of the row kernels (256 threads)
+    int P[3];  // pencils per tile of the strided kernels at width index 0, 1, 2 (0: not available)
+    int T[3];  // threads per workgroup at width index 0, 1, 2
     // strided-axis kernels, [..][pw index]
     void (*deriv[3][3])(dim3 grid, hipStream_t st, const StridedArgs& a);  // [MODE_*]
     void (*cfft[2][3])(dim3 grid, hipStream_t st, const CStridedArgs& a);  // [0 forward, 1 inverse]
