@@ -104,8 +104,8 @@ int qvti_forward_batch(qvti_ctx *c, int nt, const float *rik, const qvti_shot *s
 
 /* Shots per launch for nshots shots on this context: tuned by timing silent
  * runs of widths 1, 2, 4, ... up to min(max_shots, nshots) on the context
- * (which ends any run in progress), once per grid, medium, k-space mode,
- * capacity and shot count, and kept for the process.  Results do not depend
+ * (which ends any run in progress), once per grid, k-space mode and
+ * min(capacity, shot count), and kept for the process.  Results do not depend
  * on the width.  Writes the width; the widths tried and their ms per
  * shot-step go to trial_width / trial_ms (up to max_trials entries, either
  * may be NULL).  Returns the number of widths tried.  No counterpart in the

@@ -1278,10 +1278,10 @@ extern "C" int qvti_forward(qvti_ctx* c, int nt, const float* rik, const qvti_sh
 }
 
 // ------------------------------------------------------- batch width ----
-// Shots per launch for batched runs, tuned once per (grid, medium, k-space
-// mode, context capacity, shot count) and kept for the process: the results
+// Shots per launch for batched runs, tuned once per (grid, k-space mode,
+// min(context capacity, shot count)) and kept for the process: the results
 // do not depend on it (every shot's arithmetic is the same in any batch), only
-// the throughput does.
+// the throughput does (the medium changes coefficient values, not the work).
 namespace {
 struct WidthKey {
     int nx, ny, nz, vti, ks, cap;
@@ -1306,17 +1306,20 @@ static int time_width(qvti_ctx* c, const qvti_shot* shots, int nshots, int W, do
     std::vector<float> silent(warm + timed, 0.f);
     std::vector<qvti_shot> sh(W);
     for (int s = 0; s < W; ++s) sh[s] = shots[s % nshots];
-    int rc = qvti_begin(c, warm + timed, silent.data(), sh.data(), W);
-    if (rc) return rc;
-    const int probe = c->probe_every;
-    c->probe_every = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
         hipEventDestroy(e0);
-        c->probe_every = probe;
         return qvti_fail(QVTI_E_DEVICE, "hipEventCreate failed");
     }
+    int rc = qvti_begin(c, warm + timed, silent.data(), sh.data(), W);
+    if (rc) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return rc;
+    }
+    const int probe = c->probe_every;
+    c->probe_every = 0;
     rc = qvti_step(c, warm);
     if (!rc && hipEventRecord(e0, c->stream) != hipSuccess) rc = qvti_fail(QVTI_E_DEVICE, "event record failed");
     if (!rc) rc = qvti_step(c, timed);
