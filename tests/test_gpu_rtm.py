@@ -145,3 +145,65 @@ def test_rtm_images_a_flat_reflector_at_its_depth():
         z_peak, zr - 1, zr, col.max() / np.median(col)))
     assert abs(z_peak - (zr - 0.5)) <= 2.0
     assert col.max() > 5 * np.median(col)
+
+
+def _rtm_case():
+    shot, model = small_case(20, 24, 30, 4, 40, O.SRC_EXPLOSIVE, True)
+    with _engine(shot, model) as e:
+        recs = e.forward(shot["nt"], shot["rik"], _shot(shot))
+    return shot, model, recs
+
+
+def test_rtm_segments_do_not_change_the_image():
+    """The source wavefield recomputed from checkpoints in segments of 7 steps
+    (the last one partial) is the uninterrupted run bit for bit, so the
+    cross-correlation image and illumination equal those of one segment
+    covering all nt steps, bitwise."""
+    shot, model, recs = _rtm_case()
+    out = []
+    for ckpt in (shot["nt"], 7):
+        with _engine(shot, model) as es, _engine(shot, model) as er:
+            out.append(qvti.rtm(es, er, shot["nt"], shot["rik"], _shot(shot), [None, None, recs[2]], qvti.IC_XCORR,
+                                qvti.FIELD_PRESSURE, ckpt, 0.0))
+    assert np.abs(out[0][0]).max() > 0
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_rtm_image_is_linear_in_the_records():
+    """The receiver wavefield is linear in the injected records and scaling
+    by 2 is exact in binary floating point: doubling every record doubles the
+    cross-correlation image bit for bit and leaves the illumination alone."""
+    shot, model, recs = _rtm_case()
+    out = []
+    for scale in (1.0, 2.0):
+        with _engine(shot, model) as es, _engine(shot, model) as er:
+            out.append(qvti.rtm(es, er, shot["nt"], shot["rik"], _shot(shot), [scale * r for r in recs],
+                                qvti.IC_XCORR, 2, 9, 0.0))
+    assert np.array_equal(out[1][0], 2 * out[0][0]) and np.array_equal(out[1][1], out[0][1])
+
+
+def test_rtm_rejects_bad_arguments():
+    """One context for both wavefields, contexts of different grids, the
+    reference's partial-correlation condition (2, not built), a checkpoint
+    interval below 1, an unknown field and no records at all are refused
+    before any device work."""
+    shot, model, recs = _rtm_case()
+    nt, rik, s = shot["nt"], shot["rik"], _shot(shot)
+    other = dict(shot, nz=shot["nz"] + 2)
+    m2 = {k: np.pad(v, ((0, 0), (0, 0), (0, 2)), mode="edge") for k, v in model.items()}
+    with _engine(shot, model) as es, _engine(shot, model) as er, _engine(other, m2) as eo:
+        bad = [
+            lambda: qvti.rtm(es, es, nt, rik, s, recs, qvti.IC_XCORR, qvti.FIELD_PRESSURE, 10, 0.0),
+            lambda: qvti.rtm(es, eo, nt, rik, s, recs, qvti.IC_XCORR, qvti.FIELD_PRESSURE, 10, 0.0),
+            lambda: qvti.rtm(es, er, nt, rik, s, recs, 2, qvti.FIELD_PRESSURE, 10, 0.0),
+            lambda: qvti.rtm(es, er, nt, rik, s, recs, qvti.IC_XCORR, qvti.FIELD_PRESSURE, 0, 0.0),
+            lambda: qvti.rtm(es, er, nt, rik, s, recs, qvti.IC_XCORR, 10, 10, 0.0),
+            lambda: qvti.rtm(es, er, nt, rik, s, [None, None, None], qvti.IC_XCORR, qvti.FIELD_PRESSURE, 10, 0.0),
+        ]
+        for k, call in enumerate(bad):
+            with pytest.raises(qvti.QvtiError) as ex:
+                call()
+            assert ex.value.code == qvti.E_ARG, (k, ex.value)
+        # the contexts stay usable
+        img, _ = qvti.rtm(es, er, nt, rik, s, recs, qvti.IC_XCORR, qvti.FIELD_PRESSURE, 10, 0.0)
+        assert np.isfinite(img).all() and np.abs(img).max() > 0
