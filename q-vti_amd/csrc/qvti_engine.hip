@@ -616,7 +616,7 @@ static int build_launch_config(qvti_ctx* c)
     sz.src_ip = c->d_src_ip;
     sz.rik = nullptr;
     sz.it = c->d_it;
-    const int R = Pz / 8;
+    const int R = c->ops[2]->Rsz;
     c->g_sz = dim3((unsigned)((c->nrows + R - 1) / R), 1, 1);
     sz.ntiles = (int)c->g_sz.x;
     sz.nshots = 1;

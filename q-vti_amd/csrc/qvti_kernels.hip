@@ -93,7 +93,7 @@ LenOps make_ops()
 {
     using G = L2<A, B, 1>;
     return LenOps{
-        A * B, A, B, G::P,
+        A * B, A, B, G::P, StressTile<G>::R,
         {pencils<A, B, kW0>(), pencils<A, B, kW1>(), pencils<A, B, kW2>()},
         {L2<A, B, kW0>::T, L2<A, B, kW1>::T, L2<A, B, kW2>::T},
         {{deriv_fn<A, B, kW0, MODE_STORE>(), deriv_fn<A, B, kW1, MODE_STORE>(), deriv_fn<A, B, kW2, MODE_STORE>()},

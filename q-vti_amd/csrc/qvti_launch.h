@@ -17,6 +17,7 @@ namespace qvti {
 struct LenOps {
     int n, n1, n2;
     int Prow;  // rows per workgroup of the row kernels (256 threads)
+    int Rsz;   // rows per workgroup of the stress update (StressTile)
     int P[3];  // pencils per tile of the strided kernels at width index 0, 1, 2 (0: not available)
     int T[3];  // threads per workgroup at width index 0, 1, 2
     // strided-axis kernels, [..][pw index]

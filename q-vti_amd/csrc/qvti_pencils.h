@@ -891,6 +891,22 @@ struct StressZArgs {
     const int* it;
 };
 
+// Rows of the stress update's workgroup tile: 8 pencils (kinds) per row.  One
+// pass of G::P pencils gives P/8 rows; where that leaves a third or more of
+// the pencil slots unused (P = 12..15) and 2P/8 rows still fit 54 KB of LDS
+// (3 workgroups per CU), the first transform level takes two passes over
+// 2P/8 rows instead (n = 200 = 10 x 20: 3 rows instead of 1).
+template <class G>
+struct StressTile {
+    static constexpr int P1 = kThreads / G::N2, P2 = kThreads / G::N1;  // pencils per pass of each level
+    static constexpr int R2 = 2 * G::P / 8;
+    static constexpr bool kWide = G::P / 8 == 1 && G::P >= 12 && 8L * (8 * R2 * G::PS + G::TWN) <= 54L * 1024 &&
+                                  8 * R2 <= P2;
+    static constexpr int R = kWide ? R2 : G::P / 8;
+    static constexpr int RD1 = (8 * R + P1 - 1) / P1;  // passes of the first level
+    static_assert(8 * R <= P2, "the second level takes one pass");
+};
+
 // pencil kind -> (component, family member of the real part, of the imaginary part)
 // staging field index = 2*kind (+1 for the imaginary part)
 //   0: A1 vx + i A1 dvx (D-)   1: A1 vy + i A1 dvy (D-)   2: A1 vz + i A1 dvz (D+)
@@ -900,49 +916,54 @@ template <class G, bool VTI>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_STRESSZ_WAVES))) void k2_stress_z(
     StressZArgs a)
 {
-    constexpr int R = G::P / 8;
+    using T = StressTile<G>;
+    constexpr int R = T::R, NP = 8 * R;
     constexpr int n = G::N;
     static_assert(R >= 1, "tile too small");
     static_assert(G::PS >= n + 1, "staging must fit in the pencil buffer");
-    __shared__ float2 lds[8 * R * G::PS];
+    __shared__ float2 lds[NP * G::PS];
     __shared__ float2 tws[G::TWN];
     const TileShot ts = tile_shot(a.ntiles, a.nshots, a.run);
     if (ts.tile < 0) return;  // padding workgroup (whole workgroup, before any barrier)
     const int tid = threadIdx.x, shot = ts.shot;
-    const int j = tid % G::N2, q1 = tid / G::N2;
+    // level 1: N2 lanes per pencil, T::P1 pencils per pass; level 2: N1 lanes
+    // per pencil, all NP pencils in one pass
+    const int j = tid % G::N2, u1 = tid / G::N2;
     const int k1 = tid % G::N1, q2 = tid / G::N1;
-    const bool act1 = q1 < 8 * R, act2 = q2 < 8 * R;
+    const bool act2 = q2 < NP;
     const long long r0 = (long long)ts.tile * R;
-    const long long row1 = r0 + q1 / 8;
-    const bool ok1 = act1 && row1 < a.nrows;
     const long long soff = (long long)shot * a.shot_stride;
-    float2* pen1 = lds + q1 * G::PS;
     float2* pen2 = lds + q2 * G::PS;
     load_twiddles<G>(tws, a.tw);
     __syncthreads();
-    float2 d[G::N1];
+    float2 d[T::RD1][G::N1];
     {
-        const int kind = q1 % 8;
-        if (act1) {
-            const float* fre;
-            const float* fim;
-            switch (kind) {
-            case 0: fre = a.fam[0][0]; fim = a.fam[0][3]; break;
-            case 1: fre = a.fam[1][0]; fim = a.fam[1][3]; break;
-            case 2: fre = a.fam[2][0]; fim = a.fam[2][3]; break;
-            case 3: fre = a.fam[0][1]; fim = a.fam[0][4]; break;
-            case 4: fre = a.fam[1][1]; fim = a.fam[1][4]; break;
-            case 5: fre = a.fam[2][1]; fim = a.fam[2][4]; break;
-            case 6: fre = a.fam[0][2]; fim = a.fam[1][2]; break;
-            default: fre = a.fam[2][2]; fim = nullptr; break;
-            }
-            float2 v[G::N1];
 #pragma unroll
-            for (int j1 = 0; j1 < G::N1; ++j1) {
-                const long long g = soff + row1 * n + j1 * G::N2 + j;
-                v[j1] = ok1 ? make_float2(fre[g], fim ? fim[g] : 0.f) : make_float2(0.f, 0.f);
+        for (int r = 0; r < T::RD1; ++r) {
+            const int q1 = u1 + r * T::P1;
+            if (u1 < T::P1 && q1 < NP) {
+                const long long row1 = r0 + q1 / 8;
+                const bool ok1 = row1 < a.nrows;
+                const float* fre;
+                const float* fim;
+                switch (q1 % 8) {
+                case 0: fre = a.fam[0][0]; fim = a.fam[0][3]; break;
+                case 1: fre = a.fam[1][0]; fim = a.fam[1][3]; break;
+                case 2: fre = a.fam[2][0]; fim = a.fam[2][3]; break;
+                case 3: fre = a.fam[0][1]; fim = a.fam[0][4]; break;
+                case 4: fre = a.fam[1][1]; fim = a.fam[1][4]; break;
+                case 5: fre = a.fam[2][1]; fim = a.fam[2][4]; break;
+                case 6: fre = a.fam[0][2]; fim = a.fam[1][2]; break;
+                default: fre = a.fam[2][2]; fim = nullptr; break;
+                }
+                float2 v[G::N1];
+#pragma unroll
+                for (int j1 = 0; j1 < G::N1; ++j1) {
+                    const long long g = soff + row1 * n + j1 * G::N2 + j;
+                    v[j1] = ok1 ? make_float2(fre[g], fim ? fim[g] : 0.f) : make_float2(0.f, 0.f);
+                }
+                fwd_l1<G>(v, lds + q1 * G::PS, tws, j);
             }
-            fwd_l1<G>(v, pen1, tws, j);
         }
         __syncthreads();
         float2 b[G::N2];
@@ -956,18 +977,26 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(QVTI_S
         __syncthreads();
         if (act2) inv_la<G>(b, pen2, tws + G::T2OFF, k1);
         __syncthreads();
-        if (act1) inv_lb<G>(d, pen1, j);
+#pragma unroll
+        for (int r = 0; r < T::RD1; ++r) {
+            const int q1 = u1 + r * T::P1;
+            if (u1 < T::P1 && q1 < NP) inv_lb<G>(d[r], lds + q1 * G::PS, j);
+        }
         __syncthreads();
         // stage: st[field][rr][iz] with rows padded to n + 1 floats (odd), so
         // the rows written by one wave fall in different banks, field = 2*kind + part
         float* st = reinterpret_cast<float*>(lds);
-        if (act1) {
-            const int rr = q1 / 8;
 #pragma unroll
-            for (int j1 = 0; j1 < G::N1; ++j1) {
-                const int iz = j + G::N2 * j1;
-                st[((2 * kind) * R + rr) * (n + 1) + iz] = d[j1].x;
-                st[((2 * kind + 1) * R + rr) * (n + 1) + iz] = d[j1].y;
+        for (int r = 0; r < T::RD1; ++r) {
+            const int q1 = u1 + r * T::P1;
+            if (u1 < T::P1 && q1 < NP) {
+                const int kind = q1 % 8, rr = q1 / 8;
+#pragma unroll
+                for (int j1 = 0; j1 < G::N1; ++j1) {
+                    const int iz = j + G::N2 * j1;
+                    st[((2 * kind) * R + rr) * (n + 1) + iz] = d[r][j1].x;
+                    st[((2 * kind + 1) * R + rr) * (n + 1) + iz] = d[r][j1].y;
+                }
             }
         }
         __syncthreads();

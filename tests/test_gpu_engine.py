@@ -106,6 +106,10 @@ CASES = [
     (250, 18, 16, 4, 10, O.SRC_FY, False),
     (30, 512, 16, 4, 10, O.SRC_FZ, True),
     (72, 150, 270, 6, 8, O.SRC_EXPLOSIVE, True),
+    # row lengths whose stress-update tile takes two passes of the first
+    # transform level (200 = 10 x 20 and 162 = 9 x 18: 3 rows per workgroup)
+    (16, 18, 200, 4, 12, O.SRC_FZ, True),
+    (18, 16, 162, 4, 12, O.SRC_EXPLOSIVE, False),
 ]
 
 
