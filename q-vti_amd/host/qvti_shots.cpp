@@ -20,6 +20,17 @@
 // Shots are stepped one after another on the chosen GPU; the reference's
 // MPI shot partitioning (:236-256) becomes one process per GPU with
 // --shot-begin/--shot-step.
+//
+// --rtm-type T migrates every shot after modelling it: the mode the
+// reference declares with RTM_type, imaging_condition, direcr_or_not and
+// check_steps (:42-65, 223-229) and never builds.  T = 1 Q-compensated
+// (both wavefields propagated with the loss turned into gain), 2 with the
+// viscoelastic scheme, 3 lossless; --imaging-condition 1 EXA, 3 source-
+// normalised, 4 cross-correlation (2, partial correlation, is not built);
+// --direct-vp V removes the direct wave by subtracting a run in a
+// homogeneous vp = V model; --mig-vp FILE migrates in another interior vp
+// model (default: the modelling one).  The image and illumination of the
+// total grid are written as raw FP32 [ntx][nty][ntz].
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -43,6 +54,10 @@ struct Options {
     float kspace_vref = 0.f;
     std::string out = ".";
     int device = 0, shot_begin = 0, shot_step = 1;
+    // migration (0: modelling only)
+    int rtm_type = 0, imaging_condition = 4, checkpoint = 500;
+    float direct_vp = 0.f, rtm_eps = 0.01f;
+    std::string mig_vp_file;
 };
 
 void usage(const char* prog)
@@ -55,8 +70,12 @@ void usage(const char* prog)
         "          --vs-ratio 1.5 --qp 200 --qs 150 --rho 1800 [--eps FILE --delta FILE (VTI)]\n"
         "  scheme  --kspace-vref V (opt-in k-space correction; 0 = the reference scheme)\n"
         "  run     --out DIR --device 0 --shot-begin 0 --shot-step 1\n"
+        "  rtm     --rtm-type 0|1|2|3 (0: modelling only) --imaging-condition 4 --checkpoint 500\n"
+        "          --rtm-eps 0.01 --direct-vp 0 (homogeneous vp for direct-wave removal) --mig-vp FILE\n"
         "Writes DIR/vx_shot<is>.bin, vy_..., vz_... (raw FP32 [nt][rnx_max*rny_max]) and\n"
-        "DIR/source_xy.txt.  The total grid nx+2*pml, ... must be a supported FFT length.\n",
+        "DIR/source_xy.txt; with --rtm-type, DIR/image_shot<is>.bin, illum_shot<is>.bin and\n"
+        "image_stack.bin (raw FP32 [ntx][nty][ntz]).  The total grid nx+2*pml, ... must be a\n"
+        "supported FFT length.\n",
         prog);
 }
 
@@ -103,6 +122,12 @@ bool parse(int argc, char** argv, Options& o)
         else if (k == "--device") I(o.device);
         else if (k == "--shot-begin") I(o.shot_begin);
         else if (k == "--shot-step") I(o.shot_step);
+        else if (k == "--rtm-type") I(o.rtm_type);
+        else if (k == "--imaging-condition") I(o.imaging_condition);
+        else if (k == "--checkpoint") I(o.checkpoint);
+        else if (k == "--rtm-eps") F(o.rtm_eps);
+        else if (k == "--direct-vp") F(o.direct_vp);
+        else if (k == "--mig-vp") o.mig_vp_file = v;
         else {
             std::fprintf(stderr, "unknown option %s\n", k.c_str());
             return false;
@@ -110,6 +135,19 @@ bool parse(int argc, char** argv, Options& o)
     }
     if (o.nx < 1 || o.ny < 1 || o.nz < 1 || o.nt < 1 || o.dr < 1 || o.shot_step < 1) {
         std::fprintf(stderr, "--nx, --ny, --nz, --nt and --dr must be positive\n");
+        return false;
+    }
+    if (o.rtm_type < 0 || o.rtm_type > 3) {
+        std::fprintf(stderr, "--rtm-type must be 0 (modelling only), 1, 2 or 3\n");
+        return false;
+    }
+    if (o.imaging_condition != 1 && o.imaging_condition != 3 && o.imaging_condition != 4) {
+        std::fprintf(stderr, "--imaging-condition must be 1 (EXA), 3 (source-normalised) or 4 (cross-correlation);"
+                             " 2 (partial correlation) is not built\n");
+        return false;
+    }
+    if (o.checkpoint < 1 || !(o.rtm_eps >= 0.f) || !(o.direct_vp >= 0.f)) {
+        std::fprintf(stderr, "--checkpoint must be positive, --rtm-eps and --direct-vp non-negative\n");
         return false;
     }
     if (o.eps_file.empty() != o.delta_file.empty()) {
@@ -207,6 +245,37 @@ int main(int argc, char** argv)
         rc = fail("VTI model");
     if (!rc && o.kspace_vref > 0.f && qvti_set_kspace(ctx, o.kspace_vref, 0.f, 0.f, 0.f)) rc = fail("k-space mode");
 
+    // migration contexts (source and receiver wavefields) on the migration
+    // model, and the direct-wave context on a homogeneous one
+    qvti_ctx *msrc = nullptr, *mrcv = nullptr, *dctx = nullptr;
+    std::vector<float> stack;
+    auto model_on = [&](qvti_ctx* c, int media, const std::vector<float>& vp) -> int {
+        std::vector<float> vs(ntp);
+        for (long long ip = 0; ip < ntp; ++ip) vs[ip] = vp[ip] / o.vs_ratio;
+        if (qvti_set_media(c, media)) return fail("medium");
+        if (qvti_set_model(c, vp.data(), gp.data(), vs.data(), gs.data(), rho.data(), vti ? eps.data() : nullptr,
+                           vti ? del.data() : nullptr))
+            return fail("model");
+        if (o.kspace_vref > 0.f && qvti_set_kspace(c, o.kspace_vref, 0.f, 0.f, 0.f)) return fail("k-space mode");
+        return 0;
+    };
+    if (!rc && o.rtm_type > 0) {
+        // RTM_type 1: Q-compensated; 2: viscoelastic; 3: lossless (viscoelas_3D.cpp:42-45)
+        const int media = o.rtm_type == 1 ? QVTI_MEDIA_QCOMP : (o.rtm_type == 2 ? QVTI_MEDIA_VISCO : QVTI_MEDIA_LOSSLESS);
+        std::vector<float> mvp = velp;
+        if (!o.mig_vp_file.empty() && qvti_load_model_raw(o.mig_vp_file.c_str(), ntx, nty, ntz, pml, mvp.data()))
+            rc = fail("migration vp model");
+        if (!rc && (qvti_create(&msrc, &g, 1, o.device % ngpu) || qvti_create(&mrcv, &g, 1, o.device % ngpu)))
+            rc = fail("migration contexts");
+        if (!rc) rc = model_on(msrc, media, mvp);
+        if (!rc) rc = model_on(mrcv, media, mvp);
+        if (!rc && o.direct_vp > 0.f) {
+            if (qvti_create(&dctx, &g, 1, o.device % ngpu)) rc = fail("direct-wave context");
+            if (!rc) rc = model_on(dctx, QVTI_MEDIA_VISCO, std::vector<float>(ntp, o.direct_vp));
+        }
+        stack.assign(ntp, 0.f);
+    }
+
     // forward modelling, shot by shot (the entry point takes the whole source
     // array and the shot index, as cuda_forward_acoustic_3D does)
     std::vector<float> rec1((size_t)o.nt * rnmax), rec2(rec1.size()), rec3(rec1.size());
@@ -236,7 +305,46 @@ int main(int argc, char** argv)
         }
         std::printf("shot %d: %.3f s, %.3f Gcell-steps/s\n", is, s, (double)ntp * o.nt / s / 1e9);
         std::fflush(stdout);
+        if (!rc && o.rtm_type > 0) {
+            // the shot as the modelling ran it: x-force at s_iz + 10, receivers at r_iz
+            std::vector<int> rz(rnmax, ss[is].r_iz);
+            const qvti_shot sh{QVTI_SRC_FX, ss[is].s_ix, ss[is].s_iy, ss[is].s_iz + 10, rnmax, r_ix.data(),
+                               r_iy.data(), rz.data()};
+            if (dctx) {
+                std::vector<float> d1(rec1.size()), d2(rec1.size()), d3(rec1.size());
+                if (qvti_forward(dctx, o.nt, rik.data(), &sh, d1.data(), d2.data(), d3.data())) {
+                    rc = fail("direct wave");
+                    break;
+                }
+                for (size_t k = 0; k < rec1.size(); ++k) {
+                    rec1[k] -= d1[k];
+                    rec2[k] -= d2[k];
+                    rec3[k] -= d3[k];
+                }
+            }
+            const auto t1 = std::chrono::steady_clock::now();
+            const qvti_rtm_opts ro{o.imaging_condition, QVTI_FIELD_PRESSURE, o.checkpoint, o.rtm_eps};
+            std::vector<float> image(ntp), illum(ntp);
+            if (qvti_rtm(msrc, mrcv, o.nt, rik.data(), &sh, rec1.data(), rec2.data(), rec3.data(), &ro, image.data(),
+                         illum.data())) {
+                rc = fail("migration");
+                break;
+            }
+            for (long long ip = 0; ip < ntp; ++ip) stack[ip] += image[ip];
+            const std::string tag = "_shot" + std::to_string(is) + ".bin";
+            if (qvti_write_raw((o.out + "/image" + tag).c_str(), image.data(), ntp) ||
+                qvti_write_raw((o.out + "/illum" + tag).c_str(), illum.data(), ntp))
+                rc = fail("image");
+            std::printf("shot %d migrated: %.3f s\n", is,
+                        std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count());
+            std::fflush(stdout);
+        }
     }
+    if (!rc && o.rtm_type > 0 && qvti_write_raw((o.out + "/image_stack.bin").c_str(), stack.data(), ntp))
+        rc = fail("image stack");
+    if (dctx) qvti_destroy(dctx);
+    if (mrcv) qvti_destroy(mrcv);
+    if (msrc) qvti_destroy(msrc);
     qvti_destroy(ctx);  // replaces cuda_Device_free
     return rc;
 }

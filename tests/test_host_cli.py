@@ -19,3 +19,11 @@ def test_help_and_argument_errors():
     r = subprocess.run([EXE, "--nx", "8", "--ny", "8", "--nz", "8", "--eps", "e.bin"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 2 and "go together" in r.stderr
+    base = [EXE, "--nx", "8", "--ny", "8", "--nz", "8"]
+    r = subprocess.run(base + ["--rtm-type", "5"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "--rtm-type must be" in r.stderr
+    r = subprocess.run(base + ["--rtm-type", "1", "--imaging-condition", "2"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "partial correlation" in r.stderr
+    r = subprocess.run(base + ["--checkpoint", "0"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "--checkpoint must be positive" in r.stderr
