@@ -55,6 +55,7 @@ struct Options {
     std::string out = ".";
     int device = 0, shot_begin = 0, shot_step = 1;
     // migration (0: modelling only)
+    int snap_every = 0, snap_field = 2;  // snapshots of the interior (the reference's disabled dump)
     int rtm_type = 0, imaging_condition = 4, checkpoint = 500;
     float direct_vp = 0.f, rtm_eps = 0.01f;
     std::string mig_vp_file;
@@ -70,10 +71,12 @@ void usage(const char* prog)
         "          --vs-ratio 1.5 --qp 200 --qs 150 --rho 1800 [--eps FILE --delta FILE (VTI)]\n"
         "  scheme  --kspace-vref V (opt-in k-space correction; 0 = the reference scheme)\n"
         "  run     --out DIR --device 0 --shot-begin 0 --shot-step 1\n"
+        "          --snap-every 0 (K > 0: interior snapshots every K steps) --snap-field 2 (0-8: vx..pyz)\n"
         "  rtm     --rtm-type 0|1|2|3 (0: modelling only) --imaging-condition 4 --checkpoint 500\n"
         "          --rtm-eps 0.01 --direct-vp 0 (homogeneous vp for direct-wave removal) --mig-vp FILE\n"
         "Writes DIR/vx_shot<is>.bin, vy_..., vz_... (raw FP32 [nt][rnx_max*rny_max]) and\n"
-        "DIR/source_xy.txt; with --rtm-type, DIR/image_shot<is>.bin, illum_shot<is>.bin and\n"
+        "DIR/source_xy.txt; with --snap-every, DIR/snap_<field>_shot<is>.bin ([nsnap][nx][ny][nz]);\n"
+        "with --rtm-type, DIR/image_shot<is>.bin, illum_shot<is>.bin and\n"
         "image_stack.bin (raw FP32 [ntx][nty][ntz]).  The total grid nx+2*pml, ... must be a\n"
         "supported FFT length.\n",
         prog);
@@ -122,6 +125,8 @@ bool parse(int argc, char** argv, Options& o)
         else if (k == "--device") I(o.device);
         else if (k == "--shot-begin") I(o.shot_begin);
         else if (k == "--shot-step") I(o.shot_step);
+        else if (k == "--snap-every") I(o.snap_every);
+        else if (k == "--snap-field") I(o.snap_field);
         else if (k == "--rtm-type") I(o.rtm_type);
         else if (k == "--imaging-condition") I(o.imaging_condition);
         else if (k == "--checkpoint") I(o.checkpoint);
@@ -135,6 +140,10 @@ bool parse(int argc, char** argv, Options& o)
     }
     if (o.nx < 1 || o.ny < 1 || o.nz < 1 || o.nt < 1 || o.dr < 1 || o.shot_step < 1) {
         std::fprintf(stderr, "--nx, --ny, --nz, --nt and --dr must be positive\n");
+        return false;
+    }
+    if (o.snap_every < 0 || o.snap_field < 0 || o.snap_field > 8) {
+        std::fprintf(stderr, "--snap-every must be non-negative and --snap-field in 0..8\n");
         return false;
     }
     if (o.rtm_type < 0 || o.rtm_type > 3) {
@@ -276,6 +285,16 @@ int main(int argc, char** argv)
         stack.assign(ntp, 0.f);
     }
 
+    // snapshots of the interior window every snap_every steps (the reference's
+    // disabled dump, cuviscoelas_3D.cu:1615-1627), written per shot
+    std::vector<float> snaps;
+    if (!rc && o.snap_every > 0) {
+        const long long nsnap = (o.nt + o.snap_every - 1) / o.snap_every;
+        snaps.resize((size_t)(nsnap * o.nx * o.ny * o.nz));
+        if (qvti_set_snapshot(ctx, o.snap_field, o.snap_every, snaps.data(), (long long)snaps.size()))
+            rc = fail("snapshots");
+    }
+
     // forward modelling, shot by shot (the entry point takes the whole source
     // array and the shot index, as cuda_forward_acoustic_3D does)
     std::vector<float> rec1((size_t)o.nt * rnmax), rec2(rec1.size()), rec3(rec1.size());
@@ -302,6 +321,12 @@ int main(int argc, char** argv)
         for (int k = 0; k < 3 && !rc; ++k) {
             const std::string path = o.out + "/" + names[k] + "_shot" + std::to_string(is) + ".bin";
             if (qvti_write_raw(path.c_str(), recs[k], (long long)o.nt * rnmax)) rc = fail("record");
+        }
+        if (!rc && !snaps.empty()) {
+            static const char* fields[9] = {"vx", "vy", "vz", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz"};
+            const std::string path =
+                o.out + "/snap_" + fields[o.snap_field] + "_shot" + std::to_string(is) + ".bin";
+            if (qvti_write_raw(path.c_str(), snaps.data(), (long long)snaps.size())) rc = fail("snapshot file");
         }
         std::printf("shot %d: %.3f s, %.3f Gcell-steps/s\n", is, s, (double)ntp * o.nt / s / 1e9);
         std::fflush(stdout);

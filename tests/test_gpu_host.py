@@ -105,3 +105,30 @@ def test_host_driver_migrates_each_shot(tmp_path):
         np.abs(ref).max(), np.linalg.norm(direct) / np.linalg.norm(recs)))
     assert np.abs(ref).max() > 0
     assert np.array_equal(image, ref) and np.array_equal(illum, ref_ill) and np.array_equal(stack, image)
+
+
+def test_host_driver_writes_interior_snapshots(tmp_path):
+    """--snap-every 10 --snap-field 5: the driver writes pzz on the interior
+    every 10 steps of each shot, [nsnap][nx][ny][nz], equal bit for bit to the
+    engine's snapshot hook on the same shot run from Python."""
+    nx, ny, nz, pml, nt, dr, dt, f0, d = 24, 12, 16, 4, 35, 2, 5e-4, 30.0, 10.0
+    args = [EXE, "--nx", nx, "--ny", ny, "--nz", nz, "--pml", pml, "--nt", nt, "--dx", d, "--dy", d, "--dz", d,
+            "--dt", dt, "--f0", f0, "--nsx", 1, "--nsx0", 8, "--nsy0", 6, "--depth", 2, "--dr", dr,
+            "--vp-const", 2800, "--out", str(tmp_path), "--snap-every", 10, "--snap-field", 5]
+    r = subprocess.run([str(a) for a in args], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    snaps = np.fromfile(str(tmp_path / "snap_pzz_shot0.bin"), np.float32).reshape(4, nx, ny, nz)
+    ntx, nty, ntz = nx + 2 * pml, ny + 2 * pml, nz + 2 * pml
+    vp = np.full((ntx, nty, ntz), 2800, np.float32)
+    R = (nx // dr) * (ny // dr)
+    rx, ry = qvti.reference_receivers(pml, nx // dr, ny // dr, dr)
+    shot = qvti.make_shot(qvti.SRC_FX, (pml + 8, pml + 6, pml + 2 + 10), rx, ry, np.full(R, pml + 2, np.int32))
+    w0 = float(np.float32(2 * np.float32(3.1415926) * np.float32(f0)))
+    with qvti.Engine(ntx, nty, ntz, pml, d, d, d, dt, f0, w0) as e:
+        e.set_model(vp, qvti.gamma_from_q(np.full(vp.shape, 200, np.float32)), (vp / np.float32(1.5)).astype(np.float32),
+                    qvti.gamma_from_q(np.full(vp.shape, 150, np.float32)), np.full(vp.shape, 1800, np.float32))
+        ref = e.set_snapshot(5, 10, nt)
+        e.forward(nt, qvti.ricker(nt, dt, f0), shot)
+    print("snapshot max |pzz| %.3e" % np.abs(ref).max())
+    assert np.abs(ref[-1]).max() > 0 and np.array_equal(snaps, ref)
