@@ -52,6 +52,7 @@ struct Options {
     float vs_ratio = 1.5f, qp = 200.f, qs = 150.f, rho = 1800.f;
     std::string eps_file, delta_file;  // VTI (an extension: the reference is isotropic)
     float kspace_vref = 0.f;
+    int media = 1;  // the reference's media_type: 1 viscoelastic, 2 dispersion-only, 3 lossless, 4 Q-compensating
     std::string out = ".";
     int device = 0, shot_begin = 0, shot_step = 1;
     // migration (0: modelling only)
@@ -70,6 +71,7 @@ void usage(const char* prog)
         "  model   --vp FILE (interior, raw FP32, ix slowest) | --vp-const 2500\n"
         "          --vs-ratio 1.5 --qp 200 --qs 150 --rho 1800 [--eps FILE --delta FILE (VTI)]\n"
         "  scheme  --kspace-vref V (opt-in k-space correction; 0 = the reference scheme)\n"
+        "          --media 1 (1 viscoelastic, 2 dispersion-only, 3 lossless, 4 Q-compensating)\n"
         "  run     --out DIR --device 0 --shot-begin 0 --shot-step 1\n"
         "          --snap-every 0 (K > 0: interior snapshots every K steps) --snap-field 2 (0-8: vx..pyz)\n"
         "  rtm     --rtm-type 0|1|2|3 (0: modelling only) --imaging-condition 4 --checkpoint 500\n"
@@ -121,6 +123,7 @@ bool parse(int argc, char** argv, Options& o)
         else if (k == "--eps") o.eps_file = v;
         else if (k == "--delta") o.delta_file = v;
         else if (k == "--kspace-vref") F(o.kspace_vref);
+        else if (k == "--media") I(o.media);
         else if (k == "--out") o.out = v;
         else if (k == "--device") I(o.device);
         else if (k == "--shot-begin") I(o.shot_begin);
@@ -140,6 +143,10 @@ bool parse(int argc, char** argv, Options& o)
     }
     if (o.nx < 1 || o.ny < 1 || o.nz < 1 || o.nt < 1 || o.dr < 1 || o.shot_step < 1) {
         std::fprintf(stderr, "--nx, --ny, --nz, --nt and --dr must be positive\n");
+        return false;
+    }
+    if (o.media < 1 || o.media > 4) {
+        std::fprintf(stderr, "--media must be 1, 2, 3 or 4\n");
         return false;
     }
     if (o.snap_every < 0 || o.snap_field < 0 || o.snap_field > 8) {
@@ -250,7 +257,10 @@ int main(int argc, char** argv)
     qvti_ctx* ctx = nullptr;
     if (qvti_create(&ctx, &g, 1, o.device % ngpu)) return fail("context");
     int rc = 0;
-    if (vti && qvti_set_model(ctx, velp.data(), gp.data(), vels.data(), gs.data(), rho.data(), eps.data(), del.data()))
+    // media_type (viscoelas_3D.cpp:57-60) -> QVTI_MEDIA_*: the modelling medium
+    if (qvti_set_media(ctx, o.media - 1)) rc = fail("medium");
+    if (!rc && vti &&
+        qvti_set_model(ctx, velp.data(), gp.data(), vels.data(), gs.data(), rho.data(), eps.data(), del.data()))
         rc = fail("VTI model");
     if (!rc && o.kspace_vref > 0.f && qvti_set_kspace(ctx, o.kspace_vref, 0.f, 0.f, 0.f)) rc = fail("k-space mode");
 

@@ -132,3 +132,35 @@ def test_host_driver_writes_interior_snapshots(tmp_path):
         e.forward(nt, qvti.ricker(nt, dt, f0), shot)
     print("snapshot max |pzz| %.3e" % np.abs(ref).max())
     assert np.abs(ref[-1]).max() > 0 and np.array_equal(snaps, ref)
+
+
+@pytest.mark.parametrize("media", [2, 3, 4])
+def test_host_driver_models_in_the_reference_media(tmp_path, media):
+    """--media 2|3|4 (the reference's media_type: dispersion-only, lossless,
+    Q-compensating): the driver's records equal the engine's in that medium,
+    run from Python on the same shot, bit for bit, and differ from the
+    viscoelastic run's."""
+    nx, ny, nz, pml, nt, dr, dt, f0, d = 24, 12, 16, 4, 30, 2, 5e-4, 30.0, 10.0
+    args = [EXE, "--nx", nx, "--ny", ny, "--nz", nz, "--pml", pml, "--nt", nt, "--dx", d, "--dy", d, "--dz", d,
+            "--dt", dt, "--f0", f0, "--nsx", 1, "--nsx0", 8, "--nsy0", 6, "--depth", 2, "--dr", dr,
+            "--vp-const", 2800, "--qp", 30, "--qs", 20, "--out", str(tmp_path), "--media", media]
+    r = subprocess.run([str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    R = (nx // dr) * (ny // dr)
+    recs = np.array([np.fromfile(str(tmp_path / ("%s_shot0.bin" % c)), np.float32).reshape(nt, R)
+                     for c in ("vx", "vy", "vz")])
+    ntx, nty, ntz = nx + 2 * pml, ny + 2 * pml, nz + 2 * pml
+    vp = np.full((ntx, nty, ntz), 2800, np.float32)
+    rx, ry = qvti.reference_receivers(pml, nx // dr, ny // dr, dr)
+    shot = qvti.make_shot(qvti.SRC_FX, (pml + 8, pml + 6, pml + 2 + 10), rx, ry, np.full(R, pml + 2, np.int32))
+    w0 = float(np.float32(2 * np.float32(3.1415926) * np.float32(f0)))
+    out = {}
+    for m in (media - 1, qvti.MEDIA_VISCO):
+        with qvti.Engine(ntx, nty, ntz, pml, d, d, d, dt, f0, w0) as e:
+            e.set_media(m)
+            e.set_model(vp, qvti.gamma_from_q(np.full(vp.shape, 30, np.float32)),
+                        (vp / np.float32(1.5)).astype(np.float32),
+                        qvti.gamma_from_q(np.full(vp.shape, 20, np.float32)), np.full(vp.shape, 1800, np.float32))
+            out[m] = e.forward(nt, qvti.ricker(nt, dt, f0), shot)
+    assert np.array_equal(recs, out[media - 1])
+    assert not np.array_equal(recs, out[qvti.MEDIA_VISCO])

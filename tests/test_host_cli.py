@@ -29,3 +29,5 @@ def test_help_and_argument_errors():
     assert r.returncode == 2 and "--checkpoint must be positive" in r.stderr
     r = subprocess.run(base + ["--snap-every", "5", "--snap-field", "9"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "--snap-field in 0..8" in r.stderr
+    r = subprocess.run(base + ["--media", "0"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "--media must be" in r.stderr
