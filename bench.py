@@ -258,15 +258,16 @@ def summarize_counters(workload, trace_dir, fetch_dir, write_dir, copy_fetch_dir
     print("FETCH_SIZE x %.4f, WRITE_SIZE x %.4f (copy probe, %d MiB each way)\n" % (fetch_factor, write_factor,
                                                                                   probe >> 20))
     print("| kernel class | rocprof avg us | bench HIP-event avg us | algorithmic MB | fetch MB | write MB "
-          "| traffic / algorithmic | algorithmic GB/s |")
-    print("|---|---|---|---|---|---|---|---|")
+          "| traffic / algorithmic | algorithmic GB/s | of 8 TB/s |")
+    print("|---|---|---|---|---|---|---|---|---|")
     tot = 0.0
     for c, k in kernels.items():
         tot += k["rocprof_avg_us"]
-        print("| %s | %.1f | %.1f | %.1f | %.1f | %.1f | %.3f | %.0f |" % (
+        gbs = k["algorithmic_bytes"] / (k["rocprof_avg_us"] * 1e3)
+        print("| %s | %.1f | %.1f | %.1f | %.1f | %.1f | %.3f | %.0f | %.3f |" % (
             c, k["rocprof_avg_us"], k["bench_event_avg_us"], k["algorithmic_bytes"] / 1e6, k["fetch_bytes"] / 1e6,
-            k["write_bytes"] / 1e6, (k["fetch_bytes"] + k["write_bytes"]) / max(k["algorithmic_bytes"], 1.0),
-            k["algorithmic_bytes"] / (k["rocprof_avg_us"] * 1e3)))
+            k["write_bytes"] / 1e6, (k["fetch_bytes"] + k["write_bytes"]) / max(k["algorithmic_bytes"], 1.0), gbs,
+            gbs / HBM_PEAK_GBS))
     print("\nsum of per-step kernel time (rocprof): %.0f us; bench wall time per step %.0f us (%.4f Gcell-steps/s)"
           % (tot, line["ms_per_step"] * 1e3, line["value"]))
 
