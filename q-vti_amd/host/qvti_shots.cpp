@@ -28,7 +28,7 @@
 // viscoelastic scheme, 3 lossless; --imaging-condition 1 EXA, 3 source-
 // normalised, 4 cross-correlation (2, partial correlation, is not built);
 // --direct-vp V removes the direct wave by subtracting a run in a
-// homogeneous vp = V model; --mig-vp FILE migrates in another interior vp
+// homogeneous vp = V model (a VTI model keeps its eps and delta); --mig-vp FILE migrates in another interior vp
 // model (default: the modelling one).  The image and illumination of the
 // total grid are written as raw FP32 [ntx][nty][ntz].
 #include <chrono>
